@@ -1,0 +1,126 @@
+// Internal declarations shared by the HIP translation units of libdc_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+#include "dc_hip.h"
+#include "device_data.h"
+
+#define DCX_CHECK(expr)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      std::fprintf(stderr, "[dc_hip] HIP error %s (%s) at %s:%d: %s\n",        \
+                   hipGetErrorName(e_), hipGetErrorString(e_), __FILE__,       \
+                   __LINE__, #expr);                                           \
+      std::exit(1);                                                            \
+    }                                                                          \
+  } while (0)
+
+#define DCX_FAIL(...)                                                          \
+  do {                                                                         \
+    std::fprintf(stderr, "[dc_hip] error: ");                                  \
+    std::fprintf(stderr, __VA_ARGS__);                                         \
+    std::fprintf(stderr, "\n");                                                \
+    std::exit(1);                                                              \
+  } while (0)
+
+namespace dcx {
+
+// Coefficient arrays in dc_precomp_vars order.
+enum { CXX, CYY, CZZ, CXY, CYZ, CXZ, V2PX, V2PZ, V2SZ, V2PN, NCOEF };
+
+struct Coef {
+  const float *c[NCOEF];
+};
+
+// Half-open box of cell coordinates.
+struct Box {
+  int64_t x0, y0, z0, x1, y1, z1;
+};
+
+// Arguments of one stencil launch over one array set.
+struct StepArgs {
+  float *pp;
+  const float *pc;
+  float *qp;
+  const float *qc;
+  Coef coef;
+  int64_t ys, zs;  // y and z strides in elements (x stride is 1)
+  Box box;         // cells to update
+  float dxx, dyy, dzz, dxy, dxz, dyz;  // 1/(dx*dx) ... formed in float
+  float dt;
+  // Fused source: when src_z >= 0, add src_amp to the new p and q of the
+  // cell at plane src_z, in-plane offset src_xy (x + y*ys).
+  int64_t src_z, src_xy;
+  float src_amp;
+  int zchunk;  // planes per workgroup (fast kernel)
+};
+
+void launch_exact(const StepArgs &a, hipStream_t s);
+void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s);
+bool fast_tiling_supported(int tile_x, int tile_y);
+void launch_add_source(float *pc, float *qc, int64_t index, float amp,
+                       hipStream_t s);
+
+// One box copy between a strided array and a packed buffer.
+struct BoxCopy {
+  float *arr;        // strided array
+  int64_t ys, zs;    // its strides
+  int64_t x0, y0, z0;
+  int64_t nx, ny, nz;
+  int64_t off;       // offset of the box in the packed buffer
+};
+
+// dir = 0: pack (array -> buffer), dir = 1: unpack (buffer -> array).
+// boxes is a device array of nbox entries; total = sum of box volumes.
+void launch_box_copy(const BoxCopy *boxes, int nbox, int64_t total, float *buf,
+                     int dir, hipStream_t s);
+// Single box, passed by value.
+void launch_box_copy1(const BoxCopy &box, float *buf, int dir, hipStream_t s);
+
+// Library state behind a dc_device_data pointer.  `abi` must stay first: the
+// caller holds &state->abi.
+struct State {
+  dc_device_data abi;
+  uint64_t magic;
+  int device;
+  hipStream_t stream;
+  bool own_stream;  // false once a driver has put the struct on its stream
+  int64_t sizes[3];
+  int64_t n;
+  int64_t ys, zs;
+  float *field_mem[4];
+  float *coef_mem;
+  float *scratch;
+  size_t scratch_n;
+  float *stage;  // device staging for the ABI halo copies
+  size_t stage_n;
+  int kernel;
+  int tile_x, tile_y, zchunk;
+  int timing_detail;
+  double last_kernel_ms;
+  hipEvent_t ev[2];
+};
+
+constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"
+
+State *state_of(dc_device_data *d);
+const State *state_of(const dc_device_data *d);
+
+// Build StepArgs for the current pointer state of `st` over `box`.
+StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
+                        float dz, float dt);
+
+// Launch the selected stencil kernel.
+void launch_stencil(const State *st, const StepArgs &a);
+
+// Move a struct onto a caller-owned stream (the struct's own stream is
+// destroyed); used by drivers that order several subdomains on one stream.
+void adopt_stream(State *st, hipStream_t s);
+
+}  // namespace dcx

@@ -1,0 +1,502 @@
+// The reference's backend ABI (include/propagate.h, include/device_data.h) on
+// one MI355X, plus the library's own dcx_* entry points for device-resident
+// time loops.  All work of one dc_device_data is ordered on one HIP stream.
+#include <cstring>
+#include <vector>
+
+#include "dc_internal.hpp"
+#include "propagate.h"
+
+namespace dcx {
+
+State *state_of(dc_device_data *d) {
+  if (d == nullptr) DCX_FAIL("null dc_device_data");
+  State *st = reinterpret_cast<State *>(d);
+  if (st->magic != kMagic)
+    DCX_FAIL("dc_device_data was not created by dc_device_data_init");
+  return st;
+}
+
+const State *state_of(const dc_device_data *d) {
+  return state_of(const_cast<dc_device_data *>(d));
+}
+
+static void check_sizes(const State *st, const size_t sizes[3]) {
+  for (int k = 0; k < 3; ++k)
+    if ((int64_t)sizes[k] != st->sizes[k])
+      DCX_FAIL("sizes (%zu,%zu,%zu) differ from the initialised (%lld,%lld,%lld)",
+               sizes[0], sizes[1], sizes[2], (long long)st->sizes[0],
+               (long long)st->sizes[1], (long long)st->sizes[2]);
+}
+
+// Which device array a caller's field token names.
+static float *field_of(const State *st, const float *token) {
+  const dc_device_data &d = st->abi;
+  if (token == d.pp || token == d.pc || token == d.qp || token == d.qc)
+    return const_cast<float *>(token);
+  DCX_FAIL("array is not one of this dc_device_data's fields");
+}
+
+StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
+                        float dz, float dt) {
+  StepArgs a{};
+  a.pp = st->abi.pp;
+  a.pc = st->abi.pc;
+  a.qp = st->abi.qp;
+  a.qc = st->abi.qc;
+  const dc_precomp_vars &pv = st->abi.precomp_vars;
+  const float *cs[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
+                            pv.ch1dyz, pv.ch1dxz, pv.v2px,   pv.v2pz,
+                            pv.v2sz,   pv.v2pn};
+  for (int k = 0; k < NCOEF; ++k) a.coef.c[k] = cs[k];
+  a.ys = st->ys;
+  a.zs = st->zs;
+  a.box = box;
+  // Formed in float as include/sample_compute.h:128-133 forms them.
+  a.dxx = 1.0f / (dx * dx);
+  a.dyy = 1.0f / (dy * dy);
+  a.dzz = 1.0f / (dz * dz);
+  a.dxy = 1.0f / (dx * dy);
+  a.dxz = 1.0f / (dx * dz);
+  a.dyz = 1.0f / (dy * dz);
+  a.dt = dt;
+  a.src_z = -1;
+  a.src_xy = -1;
+  a.src_amp = 0.0f;
+  a.zchunk = st->zchunk;
+  return a;
+}
+
+void launch_stencil(const State *st, const StepArgs &a) {
+  if (st->kernel == DCX_KERNEL_EXACT)
+    launch_exact(a, st->stream);
+  else
+    launch_fast(a, st->tile_x, st->tile_y, st->stream);
+}
+
+void adopt_stream(State *st, hipStream_t s) {
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  if (st->own_stream) DCX_CHECK(hipStreamDestroy(st->stream));
+  st->stream = s;
+  st->own_stream = false;
+}
+
+static void set_source(const State *st, StepArgs &a, long long index,
+                       float amp) {
+  if (index < 0) return;
+  if (index >= st->n) DCX_FAIL("source index %lld out of range", index);
+  a.src_z = index / st->zs;
+  a.src_xy = index % st->zs;
+  a.src_amp = amp;
+}
+
+static float *ensure(float *&buf, size_t &cap, size_t n) {
+  if (n > cap) {
+    if (buf) DCX_CHECK(hipFree(buf));
+    DCX_CHECK(hipMalloc(&buf, n * sizeof(float)));
+    cap = n;
+  }
+  return buf;
+}
+
+// Device ordinal of a rank: the number of earlier ranks on the same host,
+// modulo the device count (the rule of src/device_data.cu:18-33).
+static int select_device(const dc_process_t *p) {
+  int count = 0;
+  DCX_CHECK(hipGetDeviceCount(&count));
+  if (count <= 0) DCX_FAIL("no HIP device");
+  if (p->hostnames == nullptr || p->num_workers == 0) return 0;
+  const size_t len = 256;
+  const char *mine = p->hostnames + len * (size_t)p->rank;
+  int same = 0;
+  for (int i = 0; i < (int)p->num_workers && i != p->rank; i++)
+    if (std::strncmp(mine, p->hostnames + len * (size_t)i, len) == 0) same++;
+  return same % count;
+}
+
+}  // namespace dcx
+
+using namespace dcx;
+
+extern "C" {
+
+dc_device_data *dc_device_data_init(dc_process_t *process) {
+  if (process == nullptr) DCX_FAIL("null process");
+  State *st = static_cast<State *>(std::calloc(1, sizeof(State)));
+  if (st == nullptr)
+    DCX_FAIL("OOM: could not allocate memory for device_data in "
+             "dc_device_data_init");
+  st->magic = kMagic;
+  st->device = select_device(process);
+  DCX_CHECK(hipSetDevice(st->device));
+  DCX_CHECK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+  st->own_stream = true;
+  DCX_CHECK(hipEventCreate(&st->ev[0]));
+  DCX_CHECK(hipEventCreate(&st->ev[1]));
+  for (int k = 0; k < 3; ++k) {
+    if (process->sizes[k] < 1) DCX_FAIL("empty local grid");
+    st->sizes[k] = (int64_t)process->sizes[k];
+  }
+  st->n = st->sizes[0] * st->sizes[1] * st->sizes[2];
+  if (st->sizes[0] * st->sizes[1] >= (int64_t(1) << 31))
+    DCX_FAIL("one xy plane must hold fewer than 2^31 cells");
+  st->ys = st->sizes[0];
+  st->zs = st->sizes[0] * st->sizes[1];
+  st->kernel = DCX_KERNEL_FAST;
+  st->tile_x = 32;
+  st->tile_y = 32;
+  st->zchunk = 64;
+  const size_t bytes = (size_t)st->n * sizeof(float);
+
+  float *host_fields[4] = {process->pp, process->pc, process->qp, process->qc};
+  for (int k = 0; k < 4; ++k) {
+    DCX_CHECK(hipMalloc(&st->field_mem[k], bytes));
+    if (host_fields[k])
+      DCX_CHECK(hipMemcpyAsync(st->field_mem[k], host_fields[k], bytes,
+                               hipMemcpyHostToDevice, st->stream));
+    else
+      DCX_CHECK(hipMemsetAsync(st->field_mem[k], 0, bytes, st->stream));
+  }
+  const dc_precomp_vars &pv = process->precomp_vars;
+  const float *hc[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
+                            pv.ch1dyz, pv.ch1dxz, pv.v2px,   pv.v2pz,
+                            pv.v2sz,   pv.v2pn};
+  DCX_CHECK(hipMalloc(&st->coef_mem, bytes * NCOEF));
+  float *dc[NCOEF];
+  for (int k = 0; k < NCOEF; ++k) {
+    if (hc[k] == nullptr) DCX_FAIL("precomp_vars array %d is NULL", k);
+    dc[k] = st->coef_mem + (size_t)k * st->n;
+    DCX_CHECK(hipMemcpyAsync(dc[k], hc[k], bytes, hipMemcpyHostToDevice,
+                             st->stream));
+  }
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+
+  dc_device_data &d = st->abi;
+  d.pp = st->field_mem[0];
+  d.pc = st->field_mem[1];
+  d.qp = st->field_mem[2];
+  d.qc = st->field_mem[3];
+  d.vpz = nullptr;
+  d.vsv = nullptr;
+  d.precomp_vars = dc_precomp_vars{dc[0], dc[1], dc[2], dc[3], dc[4],
+                                   dc[5], dc[6], dc[7], dc[8], dc[9]};
+  return &d;
+}
+
+void dc_device_data_free(dc_device_data *data) {
+  if (data == nullptr) return;
+  State *st = state_of(data);
+  DCX_CHECK(hipSetDevice(st->device));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  for (int k = 0; k < 4; ++k) DCX_CHECK(hipFree(st->field_mem[k]));
+  DCX_CHECK(hipFree(st->coef_mem));
+  if (st->scratch) DCX_CHECK(hipFree(st->scratch));
+  if (st->stage) DCX_CHECK(hipFree(st->stage));
+  DCX_CHECK(hipEventDestroy(st->ev[0]));
+  DCX_CHECK(hipEventDestroy(st->ev[1]));
+  if (st->own_stream) DCX_CHECK(hipStreamDestroy(st->stream));
+  st->magic = 0;
+  std::free(st);
+}
+
+void dc_device_data_get_results(dc_process_t *process, dc_device_data *data) {
+  State *st = state_of(data);
+  const size_t bytes = (size_t)st->n * sizeof(float);
+  if (process->pc == nullptr || process->qc == nullptr)
+    DCX_FAIL("process->pc/qc must be host buffers of the local size");
+  DCX_CHECK(hipMemcpyAsync(process->pc, data->pc, bytes,
+                           hipMemcpyDeviceToHost, st->stream));
+  DCX_CHECK(hipMemcpyAsync(process->qc, data->qc, bytes,
+                           hipMemcpyDeviceToHost, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void dc_device_swap_arrays(dc_device_data *data) {
+  state_of(data);
+  float *t = data->pp;
+  data->pp = data->pc;
+  data->pc = t;
+  t = data->qp;
+  data->qp = data->qc;
+  data->qc = t;
+}
+
+void dc_device_add_source(dc_device_data *data, size_t index, float source) {
+  State *st = state_of(data);
+  if ((int64_t)index >= st->n) DCX_FAIL("source index %zu out of range", index);
+  launch_add_source(data->pc, data->qc, (int64_t)index, source, st->stream);
+}
+
+static BoxCopy box_of(const State *st, const size_t start[3],
+                      const size_t end[3], float *arr) {
+  for (int k = 0; k < 3; ++k)
+    if (start[k] > end[k] || (int64_t)end[k] > st->sizes[k])
+      DCX_FAIL("box [%zu,%zu)x[%zu,%zu)x[%zu,%zu) outside the local grid",
+               start[0], end[0], start[1], end[1], start[2], end[2]);
+  BoxCopy b{};
+  b.arr = arr;
+  b.ys = st->ys;
+  b.zs = st->zs;
+  b.x0 = (int64_t)start[0];
+  b.y0 = (int64_t)start[1];
+  b.z0 = (int64_t)start[2];
+  b.nx = (int64_t)(end[0] - start[0]);
+  b.ny = (int64_t)(end[1] - start[1]);
+  b.nz = (int64_t)(end[2] - start[2]);
+  b.off = 0;
+  return b;
+}
+
+void dc_device_extract_halo_face(dc_device_data *data, float *buffer,
+                                 const size_t start_coords[DIMENSIONS],
+                                 const size_t end_coords[DIMENSIONS],
+                                 const size_t sizes[DIMENSIONS],
+                                 const float *from_array) {
+  State *st = state_of(data);
+  check_sizes(st, sizes);
+  const BoxCopy b = box_of(st, start_coords, end_coords, field_of(st, from_array));
+  const size_t n = (size_t)(b.nx * b.ny * b.nz);
+  if (n == 0) return;
+  float *stage = ensure(st->stage, st->stage_n, n);
+  launch_box_copy1(b, stage, 0, st->stream);
+  DCX_CHECK(hipMemcpyAsync(buffer, stage, n * sizeof(float),
+                           hipMemcpyDeviceToHost, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void dc_device_insert_halo_face(dc_device_data *data, const float *buffer,
+                                const size_t start_coords[DIMENSIONS],
+                                const size_t end_coords[DIMENSIONS],
+                                const size_t sizes[DIMENSIONS],
+                                float *to_array) {
+  State *st = state_of(data);
+  check_sizes(st, sizes);
+  const BoxCopy b = box_of(st, start_coords, end_coords, field_of(st, to_array));
+  const size_t n = (size_t)(b.nx * b.ny * b.nz);
+  if (n == 0) return;
+  // The staging buffer may still feed an earlier queued unpack: wait first.
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  float *stage = ensure(st->stage, st->stage_n, n);
+  DCX_CHECK(hipMemcpyAsync(stage, buffer, n * sizeof(float),
+                           hipMemcpyHostToDevice, st->stream));
+  // The caller may free `buffer` as soon as this returns.
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  launch_box_copy1(b, stage, 1, st->stream);
+}
+
+void dc_propagate(const size_t start_coords[DIMENSIONS],
+                  const size_t end_coords[DIMENSIONS],
+                  const size_t sizes[DIMENSIONS],
+                  const int process_coordinates[DIMENSIONS],
+                  const int topology[DIMENSIONS], dc_device_data *data,
+                  const float dx, const float dy, const float dz,
+                  const float dt) {
+  (void)process_coordinates;
+  (void)topology;
+  State *st = state_of(data);
+  check_sizes(st, sizes);
+  for (int k = 0; k < 3; ++k)
+    if (end_coords[k] <= start_coords[k]) return;
+  for (int k = 0; k < 3; ++k)
+    if (start_coords[k] < STENCIL || (int64_t)end_coords[k] > st->sizes[k] - STENCIL)
+      DCX_FAIL("dc_propagate box [%zu,%zu)x[%zu,%zu)x[%zu,%zu) must lie in "
+               "[%d, size-%d) on every axis",
+               start_coords[0], end_coords[0], start_coords[1], end_coords[1],
+               start_coords[2], end_coords[2], STENCIL, STENCIL);
+  const Box box{(int64_t)start_coords[0], (int64_t)start_coords[1],
+                (int64_t)start_coords[2], (int64_t)end_coords[0],
+                (int64_t)end_coords[1], (int64_t)end_coords[2]};
+  launch_stencil(st, make_step_args(st, box, dx, dy, dz, dt));
+}
+
+int dcx_set_kernel(dc_device_data *data, int kernel) {
+  State *st = state_of(data);
+  if (kernel != DCX_KERNEL_FAST && kernel != DCX_KERNEL_EXACT)
+    DCX_FAIL("unknown kernel %d", kernel);
+  const int prev = st->kernel;
+  st->kernel = kernel;
+  return prev;
+}
+
+int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk) {
+  State *st = state_of(data);
+  if (!fast_tiling_supported(tile_x, tile_y) || z_chunk < 1) return -1;
+  st->tile_x = tile_x;
+  st->tile_y = tile_y;
+  st->zchunk = z_chunk;
+  return 0;
+}
+
+void dcx_synchronize(dc_device_data *data) {
+  DCX_CHECK(hipStreamSynchronize(state_of(data)->stream));
+}
+
+static float *field_index(State *st, int which) {
+  switch (which) {
+    case 0: return st->abi.pp;
+    case 1: return st->abi.pc;
+    case 2: return st->abi.qp;
+    case 3: return st->abi.qc;
+  }
+  DCX_FAIL("field index %d not in 0..3", which);
+}
+
+void dcx_download(dc_device_data *data, int which, float *host) {
+  State *st = state_of(data);
+  DCX_CHECK(hipMemcpyAsync(host, field_index(st, which),
+                           (size_t)st->n * sizeof(float),
+                           hipMemcpyDeviceToHost, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void dcx_upload(dc_device_data *data, int which, const float *host) {
+  State *st = state_of(data);
+  DCX_CHECK(hipMemcpyAsync(field_index(st, which), host,
+                           (size_t)st->n * sizeof(float),
+                           hipMemcpyHostToDevice, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+static Box full_box(const State *st) {
+  return Box{STENCIL,
+             STENCIL,
+             STENCIL,
+             st->sizes[0] - STENCIL,
+             st->sizes[1] - STENCIL,
+             st->sizes[2] - STENCIL};
+}
+
+void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
+              long long src_index, float src_next, int add_next) {
+  State *st = state_of(data);
+  StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
+  if (add_next) set_source(st, a, src_index, src_next);
+  launch_stencil(st, a);
+  dc_device_swap_arrays(data);
+}
+
+void dcx_set_timing_detail(dc_device_data *data, int enabled) {
+  state_of(data)->timing_detail = enabled != 0;
+}
+
+double dcx_last_kernel_ms(const dc_device_data *data) {
+  return state_of(data)->last_kernel_ms;
+}
+
+double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
+               unsigned int n_steps, long long src_index, const float *src_amp,
+               int use_graph) {
+  State *st = state_of(data);
+  const bool src = src_index >= 0 && src_amp != nullptr;
+  if (src && src_index >= st->n) DCX_FAIL("source index out of range");
+  // Step it writes the fields that step it+1 reads as pc/qc; the source of
+  // step it+1 is added as they are written (see dcx_step).
+  auto enqueue = [&](std::vector<hipEvent_t> *evs) {
+    if (src && n_steps > 0)
+      launch_add_source(st->abi.pc, st->abi.qc, src_index, src_amp[0],
+                        st->stream);
+    for (unsigned it = 0; it < n_steps; ++it) {
+      StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
+      if (src && it + 1 < n_steps) set_source(st, a, src_index, src_amp[it + 1]);
+      if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it], st->stream));
+      launch_stencil(st, a);
+      if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it + 1], st->stream));
+      dc_device_swap_arrays(data);
+    }
+  };
+  float ms = 0.f;
+  if (use_graph) {
+    // Capture the whole loop, then replay it once.  Pointer swaps happen on
+    // the host during capture, so restore them before the replay's effect is
+    // accounted: the captured graph already encodes them.
+    dc_device_data before = st->abi;
+    hipGraph_t g;
+    hipGraphExec_t ge;
+    DCX_CHECK(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    enqueue(nullptr);
+    DCX_CHECK(hipStreamEndCapture(st->stream, &g));
+    DCX_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    dc_device_data after = st->abi;
+    st->abi = before;
+    DCX_CHECK(hipEventRecord(st->ev[0], st->stream));
+    DCX_CHECK(hipGraphLaunch(ge, st->stream));
+    DCX_CHECK(hipEventRecord(st->ev[1], st->stream));
+    DCX_CHECK(hipEventSynchronize(st->ev[1]));
+    st->abi = after;
+    DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
+    DCX_CHECK(hipGraphExecDestroy(ge));
+    DCX_CHECK(hipGraphDestroy(g));
+  } else if (st->timing_detail && n_steps > 0) {
+    std::vector<hipEvent_t> evs(2 * (size_t)n_steps);
+    for (auto &e : evs) DCX_CHECK(hipEventCreate(&e));
+    DCX_CHECK(hipEventRecord(st->ev[0], st->stream));
+    enqueue(&evs);
+    DCX_CHECK(hipEventRecord(st->ev[1], st->stream));
+    DCX_CHECK(hipEventSynchronize(st->ev[1]));
+    DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
+    double sum = 0.0;
+    for (unsigned it = 0; it < n_steps; ++it) {
+      float k = 0.f;
+      DCX_CHECK(hipEventElapsedTime(&k, evs[2 * it], evs[2 * it + 1]));
+      sum += k;
+    }
+    st->last_kernel_ms = sum / n_steps;
+    for (auto &e : evs) DCX_CHECK(hipEventDestroy(e));
+  } else {
+    DCX_CHECK(hipEventRecord(st->ev[0], st->stream));
+    enqueue(nullptr);
+    DCX_CHECK(hipEventRecord(st->ev[1], st->stream));
+    DCX_CHECK(hipEventSynchronize(st->ev[1]));
+    DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
+  }
+  return (double)ms;
+}
+
+float *dcx_scratch(dc_device_data *data, size_t n_floats) {
+  State *st = state_of(data);
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  return ensure(st->scratch, st->scratch_n, n_floats);
+}
+
+static void boxes_of(State *st, int nbox, const size_t *starts,
+                     const size_t *ends, float *arr, std::vector<BoxCopy> &out,
+                     int64_t &total) {
+  out.resize(nbox);
+  total = 0;
+  for (int k = 0; k < nbox; ++k) {
+    out[k] = box_of(st, starts + 3 * k, ends + 3 * k, arr);
+    out[k].off = total;
+    total += out[k].nx * out[k].ny * out[k].nz;
+  }
+}
+
+static void copy_boxes(State *st, float *dev_buf, int nbox, const size_t *starts,
+                       const size_t *ends, float *arr, int dir) {
+  std::vector<BoxCopy> boxes;
+  int64_t total = 0;
+  boxes_of(st, nbox, starts, ends, arr, boxes, total);
+  if (total == 0) return;
+  BoxCopy *d_boxes = nullptr;
+  DCX_CHECK(hipMallocAsync((void **)&d_boxes, nbox * sizeof(BoxCopy), st->stream));
+  DCX_CHECK(hipMemcpyAsync(d_boxes, boxes.data(), nbox * sizeof(BoxCopy),
+                           hipMemcpyHostToDevice, st->stream));
+  launch_box_copy(d_boxes, nbox, total, dev_buf, dir, st->stream);
+  DCX_CHECK(hipFreeAsync(d_boxes, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void dcx_pack_boxes(dc_device_data *data, float *dev_buf, int nbox,
+                    const size_t *starts, const size_t *ends,
+                    const float *from) {
+  State *st = state_of(data);
+  copy_boxes(st, dev_buf, nbox, starts, ends, field_of(st, from), 0);
+}
+
+void dcx_unpack_boxes(dc_device_data *data, const float *dev_buf, int nbox,
+                      const size_t *starts, const size_t *ends, float *to) {
+  State *st = state_of(data);
+  copy_boxes(st, const_cast<float *>(dev_buf), nbox, starts, ends,
+             field_of(st, to), 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,450 @@
+// Fletcher TTI time-step kernels for gfx950.
+//
+// The per-cell update is the reference's include/sample_compute.h:105-185:
+//   H1(f) = sum_ij ch1d_ij * D_ij f   (ij in xx,yy,zz,xy,xz,yz)
+//   H2(f) = Dxx f + Dyy f + Dzz f - H1(f)
+//   rhsp  = v2px*H2(p) + v2pz*H1(q) + v2sz*(H1(p)-H1(q))
+//   rhsq  = v2pn*H2(p) + v2pz*H1(q) - v2sz*(H2(p)-H2(q))
+//   pp    = 2*pc - pp + rhsp*dt*dt   (same for q)
+// with the 8th-order operators of include/derivatives.h:15-91.
+//
+// Two kernels:
+//  * k_exact: one thread per cell, the reference's expression order with FP
+//    contraction off, 64-point mixed derivatives read from global memory.
+//    Bit-identical to the reference's OpenMP path; used for parity.
+//  * k_fast: the production kernel.  A workgroup owns a TX x TY column tile
+//    and a chunk of z planes and streams through z.  Each new plane of p and q
+//    (tile + 4-cell apron) is staged in LDS; x/y derivatives come from LDS,
+//    z derivatives from 8-slot register rings.  Mixed derivatives use the
+//    exact factorisation D_ab f = D1_b(D1_a f) (the 64-point weights are the
+//    products L_a*L_b), so Dxz and Dyz are first differences over z of the
+//    per-plane D1x and D1y rings and Dxy is D1y of an LDS plane of D1x.
+//    Every input byte is read once from HBM per step apart from the apron.
+#include "dc_internal.hpp"
+
+namespace dcx {
+
+// 8th-order weights, spelled as in include/derivatives.h:15-38.
+#define W_L1 0.8f
+#define W_L2 -0.2f
+#define W_L3 0.0380952380952381f
+#define W_L4 -0.0035714285714285713f
+#define W_L11 0.64f
+#define W_L12 -0.16f
+#define W_L13 0.03047619047619047618f
+#define W_L14 -0.00285714285714285713f
+#define W_L22 0.04f
+#define W_L23 -0.00761904761904761904f
+#define W_L24 0.00071428571428571428f
+#define W_L33 0.00145124716553287981f
+#define W_L34 -0.00013605442176870748f
+#define W_L44 0.00001275510204081632f
+#define W_K0 -2.84722222222222222222f
+#define W_K1 1.6f
+#define W_K2 -0.2f
+#define W_K3 0.02539682539682539682f
+#define W_K4 -0.00178571428571428571f
+
+// ------------------------------------------------------------ exact ----
+
+__device__ __forceinline__ float ex_d2(const float *__restrict__ f, int64_t i,
+                                       int64_t s, float inv) {
+#pragma clang fp contract(off)
+  return (W_K0 * f[i] + W_K1 * (f[i + s] + f[i - s]) +
+          W_K2 * (f[i + 2 * s] + f[i - 2 * s]) +
+          W_K3 * (f[i + 3 * s] + f[i - 3 * s]) +
+          W_K4 * (f[i + 4 * s] + f[i - 4 * s])) *
+         inv;
+}
+
+// (+a along u,+b along v) quad, left to right.
+#define EX_QUAD(f, i, a, u, b, v)                                              \
+  (f[(i) + (b) * (v) + (a) * (u)] - f[(i) + (b) * (v) - (a) * (u)] -          \
+   f[(i) - (b) * (v) + (a) * (u)] + f[(i) - (b) * (v) - (a) * (u)])
+// The 8 points of one off-diagonal weight as one left-to-right chain
+// (include/derivatives.h:60-87).
+#define EX_PAIR(f, i, p, q, u, v)                                              \
+  (f[(i) + (q) * (v) + (p) * (u)] - f[(i) + (q) * (v) - (p) * (u)] -          \
+   f[(i) - (q) * (v) + (p) * (u)] + f[(i) - (q) * (v) - (p) * (u)] +          \
+   f[(i) + (p) * (v) + (q) * (u)] - f[(i) + (p) * (v) - (q) * (u)] -          \
+   f[(i) - (p) * (v) + (q) * (u)] + f[(i) - (p) * (v) - (q) * (u)])
+
+__device__ __forceinline__ float ex_dx(const float *__restrict__ f, int64_t i,
+                                       int64_t u, int64_t v, float inv) {
+#pragma clang fp contract(off)
+  return (W_L11 * EX_QUAD(f, i, 1, u, 1, v) + W_L12 * EX_PAIR(f, i, 2, 1, u, v) +
+          W_L13 * EX_PAIR(f, i, 3, 1, u, v) + W_L14 * EX_PAIR(f, i, 4, 1, u, v) +
+          W_L22 * EX_QUAD(f, i, 2, u, 2, v) + W_L23 * EX_PAIR(f, i, 3, 2, u, v) +
+          W_L24 * EX_PAIR(f, i, 4, 2, u, v) + W_L33 * EX_QUAD(f, i, 3, u, 3, v) +
+          W_L34 * EX_PAIR(f, i, 4, 3, u, v) + W_L44 * EX_QUAD(f, i, 4, u, 4, v)) *
+         inv;
+}
+
+__global__ void __launch_bounds__(256) k_exact(const StepArgs a) {
+#pragma clang fp contract(off)
+  const int64_t x = a.box.x0 + (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t y = a.box.y0 + (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int64_t z = a.box.z0 + (int64_t)blockIdx.z;
+  if (x >= a.box.x1 || y >= a.box.y1) return;
+  const int64_t i = x + y * a.ys + z * a.zs;
+  const int64_t sy = a.ys, sz = a.zs;
+  const float *__restrict__ pc = a.pc;
+  const float *__restrict__ qc = a.qc;
+  const float *const *c = a.coef.c;
+
+  const float pxx = ex_d2(pc, i, 1, a.dxx);
+  const float pyy = ex_d2(pc, i, sy, a.dyy);
+  const float pzz = ex_d2(pc, i, sz, a.dzz);
+  const float pxy = ex_dx(pc, i, 1, sy, a.dxy);
+  const float pyz = ex_dx(pc, i, sy, sz, a.dyz);
+  const float pxz = ex_dx(pc, i, 1, sz, a.dxz);
+  const float h1p = c[CXX][i] * pxx + c[CYY][i] * pyy + c[CZZ][i] * pzz +
+                    c[CXY][i] * pxy + c[CXZ][i] * pxz + c[CYZ][i] * pyz;
+  const float h2p = pxx + pyy + pzz - h1p;
+
+  const float qxx = ex_d2(qc, i, 1, a.dxx);
+  const float qyy = ex_d2(qc, i, sy, a.dyy);
+  const float qzz = ex_d2(qc, i, sz, a.dzz);
+  const float qxy = ex_dx(qc, i, 1, sy, a.dxy);
+  const float qyz = ex_dx(qc, i, sy, sz, a.dyz);
+  const float qxz = ex_dx(qc, i, 1, sz, a.dxz);
+  const float h1q = c[CXX][i] * qxx + c[CYY][i] * qyy + c[CZZ][i] * qzz +
+                    c[CXY][i] * qxy + c[CXZ][i] * qxz + c[CYZ][i] * qyz;
+  const float h2q = qxx + qyy + qzz - h1q;
+
+  const float h1pmq = h1p - h1q;
+  const float h2pmq = h2p - h2q;
+  const float rhsp = c[V2PX][i] * h2p + c[V2PZ][i] * h1q + c[V2SZ][i] * h1pmq;
+  const float rhsq = c[V2PN][i] * h2p + c[V2PZ][i] * h1q - c[V2SZ][i] * h2pmq;
+  float pn = 2.0f * pc[i] - a.pp[i] + rhsp * a.dt * a.dt;
+  float qn = 2.0f * qc[i] - a.qp[i] + rhsq * a.dt * a.dt;
+  if (z == a.src_z && x + y * a.ys == a.src_xy) {
+    pn += a.src_amp;
+    qn += a.src_amp;
+  }
+  a.pp[i] = pn;
+  a.qp[i] = qn;
+}
+
+void launch_exact(const StepArgs &a, hipStream_t s) {
+  const int64_t nx = a.box.x1 - a.box.x0, ny = a.box.y1 - a.box.y0,
+                nz = a.box.z1 - a.box.z0;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
+  dim3 grid((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4), (unsigned)nz);
+  hipLaunchKernelGGL(k_exact, grid, dim3(256), 0, s, a);
+  DCX_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------- fast ----
+
+// Pair sums/differences along one axis from LDS at index l with stride st:
+// s_a = f(+a) + f(-a), d_a = f(+a) - f(-a), a = 1..4.
+struct Pairs {
+  float s1, s2, s3, s4, d1, d2, d3, d4;
+};
+
+__device__ __forceinline__ Pairs lds_pairs(const float *L, int l, int st) {
+  const float m1 = L[l - st], p1 = L[l + st];
+  const float m2 = L[l - 2 * st], p2 = L[l + 2 * st];
+  const float m3 = L[l - 3 * st], p3 = L[l + 3 * st];
+  const float m4 = L[l - 4 * st], p4 = L[l + 4 * st];
+  return Pairs{p1 + m1, p2 + m2, p3 + m3, p4 + m4,
+               p1 - m1, p2 - m2, p3 - m3, p4 - m4};
+}
+
+__device__ __forceinline__ float d2_of(float c, const Pairs &q) {
+  return W_K0 * c + W_K1 * q.s1 + W_K2 * q.s2 + W_K3 * q.s3 + W_K4 * q.s4;
+}
+
+__device__ __forceinline__ float d1_of(const Pairs &q) {
+  return W_L1 * q.d1 + W_L2 * q.d2 + W_L3 * q.d3 + W_L4 * q.d4;
+}
+
+__device__ __forceinline__ float d1_lds(const float *L, int l, int st) {
+  return W_L1 * (L[l + st] - L[l - st]) + W_L2 * (L[l + 2 * st] - L[l - 2 * st]) +
+         W_L3 * (L[l + 3 * st] - L[l - 3 * st]) +
+         W_L4 * (L[l + 4 * st] - L[l - 4 * st]);
+}
+
+// Load/store at a wave-uniform base plus a per-lane unsigned 32-bit byte
+// offset, the form gfx950's global_load/store take with an SGPR base address
+// (one VGPR per offset instead of a 64-bit address pair per load).
+__device__ __forceinline__ float ldg(const float *base, uint32_t boff) {
+  return *reinterpret_cast<const float *>(
+      reinterpret_cast<const char *>(base) + boff);
+}
+__device__ __forceinline__ void stg(float *base, uint32_t boff, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
+  constexpr int NT = TX * TY;
+  constexpr int PW = TX + 8;       // LDS plane row pitch (4-cell apron)
+  constexpr int PH = TY + 8;
+  constexpr int PL = PW * PH;      // cells of one staged plane
+  constexpr int NAPR = PL - TX * TY;
+  constexpr int NDXA = 8 * TX;     // D1x values of the 8 apron rows
+  static_assert(NAPR <= NT, "apron must fit one cell per thread");
+  static_assert(NDXA <= NT, "D1x apron must fit one cell per thread");
+
+  // One buffer each suffices: every LDS read of plane n happens before the
+  // second barrier of step n, every write of plane n+1 after it.
+  __shared__ float sP[PL], sQ[PL];              // staged plane of p and q
+  __shared__ float sDP[TX * PH], sDQ[TX * PH];  // D1x of p and q, rows -4..TY+3
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int64_t zs = a.zs;
+  const int ys = (int)a.ys;
+  const int tx0 = (int)a.box.x0 + (int)blockIdx.x * TX;
+  const int ty0 = (int)a.box.y0 + (int)blockIdx.y * TY;
+  // Last coordinate any updated cell reads; loads are clamped to it so that a
+  // tile overhanging the box never leaves the array.
+  const int xl = (int)a.box.x1 + 3, yl = (int)a.box.y1 + 3;
+
+  const int x = tx0 + tx, y = ty0 + ty;
+  const bool act = x < a.box.x1 && y < a.box.y1;
+  // In-plane offset of this thread's cell (one plane holds < 2^31 cells) and
+  // its byte offset.
+  const int own = (x < xl ? x : xl) + (y < yl ? y : yl) * ys;
+  const uint32_t ob = (uint32_t)own * 4u;
+  const int lo = (ty + 4) * PW + tx + 4;  // own cell in the LDS plane
+  const int ld = (ty + 4) * TX + tx;      // own cell in the D1x plane
+
+  // Apron cell of this thread (top rows, bottom rows, then side columns).
+  int la = -1, oa = 0;
+  if (tid < NAPR) {
+    int ax, ay;
+    if (tid < 4 * PW) {
+      ay = tid / PW;
+      ax = tid % PW;
+    } else if (tid < 8 * PW) {
+      const int t = tid - 4 * PW;
+      ay = TY + 4 + t / PW;
+      ax = t % PW;
+    } else {
+      const int t = tid - 8 * PW;
+      ay = 4 + t / 8;
+      const int c8 = t % 8;
+      ax = c8 < 4 ? c8 : TX + c8;
+    }
+    la = ay * PW + ax;
+    int gx = tx0 + ax - 4, gy = ty0 + ay - 4;
+    gx = gx < xl ? gx : xl;
+    gy = gy < yl ? gy : yl;
+    oa = gx + gy * ys;
+  }
+  const uint32_t oab = (uint32_t)oa * 4u;
+  // D1x of the apron rows: rows 0..3 and TY+4..TY+7 of the D1x plane.
+  int dr = -1, dc = 0;
+  if (tid < NDXA) {
+    const int r = tid / TX;
+    dc = tid % TX;
+    dr = r < 4 ? r : TY + r;
+  }
+
+  const float dt2 = a.dt * a.dt;
+  // Base addresses (wave-uniform, SGPRs).
+  const float *const Cxx = a.coef.c[CXX], *const Cyy = a.coef.c[CYY],
+                     *const Cxy = a.coef.c[CXY], *const Czz = a.coef.c[CZZ],
+                     *const Cxz = a.coef.c[CXZ], *const Cyz = a.coef.c[CYZ],
+                     *const Vpx = a.coef.c[V2PX], *const Vpz = a.coef.c[V2PZ],
+                     *const Vsz = a.coef.c[V2SZ], *const Vpn = a.coef.c[V2PN];
+  const float *const Gpc = a.pc, *const Gqc = a.qc;
+  float *const Gpp = a.pp, *const Gqp = a.qp;
+  // Byte offset of plane k: every array is smaller than 4 GiB (checked at
+  // launch), so one 32-bit offset per plane serves all eleven arrays.
+  const uint32_t zsb = (uint32_t)zs * 4u;
+  const int zb = (int)a.box.z0 + (int)blockIdx.z * a.zchunk;
+  const int ze = zb + a.zchunk < (int)a.box.z1 ? zb + a.zchunk : (int)a.box.z1;
+
+  // Register rings (slot = plane index - (zb-4), mod 8 / mod 4).
+  float rp[8], rq[8], xp[8], yp[8], xq[8], yq[8];
+  float Sp[4], Hp[4], Sq[4], Hq[4];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) rp[k] = rq[k] = xp[k] = yp[k] = xq[k] = yq[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) Sp[k] = Hp[k] = Sq[k] = Hq[k] = 0.f;
+
+  // The staged plane's own and apron values are prefetched one plane ahead
+  // (they are needed first thing in each step); the coefficients and the
+  // old pp/qp are loaded at the top of the step that uses them.
+  float cp, cq, ap = 0.f, aq = 0.f;
+  {
+    const uint32_t b = (uint32_t)(zb - 4) * zsb;
+    cp = ldg(Gpc, b + ob);
+    cq = ldg(Gqc, b + ob);
+    if (la >= 0) {
+      ap = ldg(Gpc, b + oab);
+      aq = ldg(Gqc, b + oab);
+    }
+  }
+
+  const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
+  for (int nb = zb - 4; nb < nend; nb += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int n = nb + u;
+      if (n >= nend) break;
+      // Stage plane n.
+      sP[lo] = cp;
+      sQ[lo] = cq;
+      if (la >= 0) {
+        sP[la] = ap;
+        sQ[la] = aq;
+      }
+      __syncthreads();
+
+      // Loads of this step (plane n coefficients, output plane j = n-4
+      // clamped into the chunk) and prefetch of plane n+1.
+      const uint32_t bc = (uint32_t)n * zsb + ob;
+      const uint32_t bj = (uint32_t)(n - 4 > zb ? n - 4 : zb) * zsb + ob;
+      const uint32_t bn = (uint32_t)(n + 1 < nend ? n + 1 : nend - 1) * zsb;
+      const float cxx = ldg(Cxx, bc), cyy = ldg(Cyy, bc), cxy = ldg(Cxy, bc);
+      const float czz = ldg(Czz, bj), cxz = ldg(Cxz, bj), cyz = ldg(Cyz, bj),
+                  vpx = ldg(Vpx, bj), vpz = ldg(Vpz, bj), vsz = ldg(Vsz, bj),
+                  vpn = ldg(Vpn, bj), opp = ldg(Gpp, bj), oqp = ldg(Gqp, bj);
+      const float ncp = ldg(Gpc, bn + ob), ncq = ldg(Gqc, bn + ob);
+      float nap = 0.f, naq = 0.f;
+      if (la >= 0) {
+        nap = ldg(Gpc, bn + oab);
+        naq = ldg(Gqc, bn + oab);
+      }
+
+      // In-plane derivatives of plane n.
+      const Pairs pxq = lds_pairs(sP, lo, 1), pyq = lds_pairs(sP, lo, PW);
+      const Pairs qxq = lds_pairs(sQ, lo, 1), qyq = lds_pairs(sQ, lo, PW);
+      const float pxx = d2_of(cp, pxq) * a.dxx, pyy = d2_of(cp, pyq) * a.dyy;
+      const float qxx = d2_of(cq, qxq) * a.dxx, qyy = d2_of(cq, qyq) * a.dyy;
+      const float dxp = d1_of(pxq), dyp = d1_of(pyq);
+      const float dxq = d1_of(qxq), dyq = d1_of(qyq);
+      sDP[ld] = dxp;
+      sDQ[ld] = dxq;
+      if (dr >= 0) {
+        const int l = dr * PW + dc + 4;
+        sDP[dr * TX + dc] = d1_lds(sP, l, 1);
+        sDQ[dr * TX + dc] = d1_lds(sQ, l, 1);
+      }
+      __syncthreads();
+      const float pxy = d1_lds(sDP, ld, TX) * a.dxy;
+      const float qxy = d1_lds(sDQ, ld, TX) * a.dxy;
+
+      // Output plane j = n - 4 once the rings hold planes j-4 .. j+4.
+      if (n >= zb + 4) {
+        const int J = (u + 4) & 7;  // slot of plane j
+        const int O = u;            // slot of plane j-4 (about to be replaced)
+#define RS(r, b) r[(J + (b)) & 7]
+        const float pzz = (W_K0 * rp[J] + W_K1 * (RS(rp, 1) + RS(rp, 7)) +
+                           W_K2 * (RS(rp, 2) + RS(rp, 6)) +
+                           W_K3 * (RS(rp, 3) + RS(rp, 5)) +
+                           W_K4 * (cp + rp[O])) * a.dzz;
+        const float qzz = (W_K0 * rq[J] + W_K1 * (RS(rq, 1) + RS(rq, 7)) +
+                           W_K2 * (RS(rq, 2) + RS(rq, 6)) +
+                           W_K3 * (RS(rq, 3) + RS(rq, 5)) +
+                           W_K4 * (cq + rq[O])) * a.dzz;
+        const float pxz = (W_L1 * (RS(xp, 1) - RS(xp, 7)) +
+                           W_L2 * (RS(xp, 2) - RS(xp, 6)) +
+                           W_L3 * (RS(xp, 3) - RS(xp, 5)) +
+                           W_L4 * (dxp - xp[O])) * a.dxz;
+        const float pyz = (W_L1 * (RS(yp, 1) - RS(yp, 7)) +
+                           W_L2 * (RS(yp, 2) - RS(yp, 6)) +
+                           W_L3 * (RS(yp, 3) - RS(yp, 5)) +
+                           W_L4 * (dyp - yp[O])) * a.dyz;
+        const float qxz = (W_L1 * (RS(xq, 1) - RS(xq, 7)) +
+                           W_L2 * (RS(xq, 2) - RS(xq, 6)) +
+                           W_L3 * (RS(xq, 3) - RS(xq, 5)) +
+                           W_L4 * (dxq - xq[O])) * a.dxz;
+        const float qyz = (W_L1 * (RS(yq, 1) - RS(yq, 7)) +
+                           W_L2 * (RS(yq, 2) - RS(yq, 6)) +
+                           W_L3 * (RS(yq, 3) - RS(yq, 5)) +
+                           W_L4 * (dyq - yq[O])) * a.dyz;
+#undef RS
+        const int D = u & 3;
+        const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
+        const float h2p = Sp[D] + pzz - h1p;
+        const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
+        const float h2q = Sq[D] + qzz - h1q;
+        const float rhsp = vpx * h2p + vpz * h1q + vsz * (h1p - h1q);
+        const float rhsq = vpn * h2p + vpz * h1q - vsz * (h2p - h2q);
+        float pn = 2.0f * rp[J] - opp + rhsp * dt2;
+        float qn = 2.0f * rq[J] - oqp + rhsq * dt2;
+        const int j = n - 4;
+        if (j == a.src_z && (int64_t)own == a.src_xy) {
+          pn += a.src_amp;
+          qn += a.src_amp;
+        }
+        if (act) {
+          const uint32_t bo = (uint32_t)j * zsb + ob;
+          stg(Gpp, bo, pn);
+          stg(Gqp, bo, qn);
+        }
+      }
+      // Push plane n into the rings.
+      rp[u] = cp;
+      rq[u] = cq;
+      xp[u] = dxp;
+      yp[u] = dyp;
+      xq[u] = dxq;
+      yq[u] = dyq;
+      Sp[u & 3] = pxx + pyy;
+      Hp[u & 3] = cxx * pxx + cyy * pyy + cxy * pxy;
+      Sq[u & 3] = qxx + qyy;
+      Hq[u & 3] = cxx * qxx + cyy * qyy + cxy * qxy;
+
+      cp = ncp;
+      cq = ncq;
+      ap = nap;
+      aq = naq;
+    }
+  }
+}
+
+bool fast_tiling_supported(int tx, int ty) {
+  // The apron (one plane's 4-cell frame) must fit one cell per thread.
+  return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && ty == 16);
+}
+
+void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
+  const int64_t nx = a.box.x1 - a.box.x0, ny = a.box.y1 - a.box.y0,
+                nz = a.box.z1 - a.box.z0;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
+  if (a.zchunk <= 0) DCX_FAIL("z chunk must be positive");
+  // The kernel addresses every array with 32-bit byte offsets.
+  if ((a.box.z1 + 4) * a.zs * 4 > (int64_t(1) << 32) - 1)
+    DCX_FAIL("fast kernel needs arrays below 4 GiB (%lld planes of %lld cells)",
+             (long long)(a.box.z1 + 4), (long long)a.zs);
+  const int64_t gz = (nz + a.zchunk - 1) / a.zchunk;
+  if (gz > 65535) DCX_FAIL("z chunk too small for %lld planes", (long long)nz);
+  dim3 grid((unsigned)((nx + tile_x - 1) / tile_x),
+            (unsigned)((ny + tile_y - 1) / tile_y), (unsigned)gz);
+  dim3 block(tile_x * tile_y);
+#define DCX_LAUNCH(TX, TY)                                                     \
+  if (tile_x == TX && tile_y == TY) {                                          \
+    hipLaunchKernelGGL((k_fast<TX, TY>), grid, block, 0, s, a);                \
+    DCX_CHECK(hipGetLastError());                                              \
+    return;                                                                    \
+  }
+  DCX_LAUNCH(32, 16)
+  DCX_LAUNCH(32, 32)
+  DCX_LAUNCH(64, 16)
+#undef DCX_LAUNCH
+  DCX_FAIL("unsupported tile %dx%d", tile_x, tile_y);
+}
+
+// -------------------------------------------------------------- misc ----
+
+__global__ void k_add_source(float *pc, float *qc, int64_t index, float amp) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    pc[index] += amp;
+    qc[index] += amp;
+  }
+}
+
+void launch_add_source(float *pc, float *qc, int64_t index, float amp,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_add_source, dim3(1), dim3(64), 0, s, pc, qc, index, amp);
+  DCX_CHECK(hipGetLastError());
+}
+
+}  // namespace dcx

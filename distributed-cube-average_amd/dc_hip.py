@@ -1,0 +1,340 @@
+"""ctypes bindings of libdc_hip.so (the MI355X Fletcher TTI backend).
+
+Mirrors the reference's backend ABI (include/propagate.h, include/device_data.h)
+with the same names and argument meaning, plus the library's dcx_* API.  The
+structs below are field-for-field copies of include/dc_process.h and
+include/precomp.h so a dc_process_t built here is what the reference's C
+worker would hand to dc_device_data_init.
+
+Arrays are numpy float32 shaped (sz, sy, sx) -- x fastest, as in
+include/indexing.h:15-20.  Every call goes to the HIP library; there is no
+CPU fallback, and loading fails loudly if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdc_hip.so")
+
+STENCIL = 4
+NEIGHBOURHOOD = 27
+COEF_NAMES = ("ch1dxx", "ch1dyy", "ch1dzz", "ch1dxy", "ch1dyz", "ch1dxz",
+              "v2px", "v2pz", "v2sz", "v2pn")
+KERNEL_FAST = 0
+KERNEL_EXACT = 1
+
+_FP = C.POINTER(C.c_float)
+_SZ3 = C.c_size_t * 3
+_I3 = C.c_int * 3
+
+
+class PrecompVars(C.Structure):
+    """dc_precomp_vars (include/precomp.h:6-17)."""
+    _fields_ = [(n, _FP) for n in COEF_NAMES]
+
+
+class Anisotropy(C.Structure):
+    """dc_anisotropy_t (include/precomp.h:19-26)."""
+    _fields_ = [(n, _FP) for n in ("theta", "phi", "vsv", "vpz", "epsilon",
+                                   "delta")]
+
+
+class Process(C.Structure):
+    """dc_process_t (include/dc_process.h:7-21)."""
+    _fields_ = [("rank", C.c_int), ("coordinates", C.c_int * 3),
+                ("neighbours", C.c_int * NEIGHBOURHOOD),
+                ("topology", C.c_int * 3), ("iterations", C.c_uint),
+                ("source_index", C.c_int), ("dx", C.c_float),
+                ("dy", C.c_float), ("dz", C.c_float), ("dt", C.c_float),
+                ("sizes", C.c_size_t * 3), ("anisotropy_vars", Anisotropy),
+                ("precomp_vars", PrecompVars), ("pp", _FP), ("pc", _FP),
+                ("qp", _FP), ("qc", _FP), ("hostnames", C.c_char_p),
+                ("num_workers", C.c_size_t)]
+
+
+class DeviceDataStruct(C.Structure):
+    """dc_device_data (include/device_data.h:11-15)."""
+    _fields_ = [("pp", _FP), ("pc", _FP), ("qp", _FP), ("qc", _FP),
+                ("vpz", _FP), ("vsv", _FP), ("precomp_vars", PrecompVars)]
+
+
+class ProgramArgs(C.Structure):
+    """dcx_program_args (include/dc_hip.h)."""
+    _fields_ = [("size_x", C.c_size_t), ("size_y", C.c_size_t),
+                ("size_z", C.c_size_t), ("dx", C.c_float), ("dy", C.c_float),
+                ("dz", C.c_float), ("dt", C.c_float), ("time_max", C.c_float),
+                ("absorption", C.c_size_t), ("topology", C.c_int * 3),
+                ("kernel", C.c_int)]
+
+
+_DD = C.POINTER(DeviceDataStruct)
+
+# (name, restype, argtypes) of every symbol the headers in include/ declare.
+SIGNATURES = {
+    # include/propagate.h
+    "dc_propagate": (None, [_SZ3, _SZ3, _SZ3, _I3, _I3, _DD, C.c_float,
+                            C.c_float, C.c_float, C.c_float]),
+    # include/device_data.h
+    "dc_device_data_init": (_DD, [C.POINTER(Process)]),
+    "dc_device_data_free": (None, [_DD]),
+    "dc_device_data_get_results": (None, [C.POINTER(Process), _DD]),
+    "dc_device_swap_arrays": (None, [_DD]),
+    "dc_device_add_source": (None, [_DD, C.c_size_t, C.c_float]),
+    "dc_device_extract_halo_face": (None, [_DD, _FP, _SZ3, _SZ3, _SZ3, _FP]),
+    "dc_device_insert_halo_face": (None, [_DD, _FP, _SZ3, _SZ3, _SZ3, _FP]),
+    # include/dc_hip.h
+    "dcx_set_kernel": (C.c_int, [_DD, C.c_int]),
+    "dcx_set_tiling": (C.c_int, [_DD, C.c_int, C.c_int, C.c_int]),
+    "dcx_synchronize": (None, [_DD]),
+    "dcx_download": (None, [_DD, C.c_int, _FP]),
+    "dcx_upload": (None, [_DD, C.c_int, _FP]),
+    "dcx_step": (None, [_DD, C.c_float, C.c_float, C.c_float, C.c_float,
+                        C.c_longlong, C.c_float, C.c_int]),
+    "dcx_run": (C.c_double, [_DD, C.c_float, C.c_float, C.c_float, C.c_float,
+                             C.c_uint, C.c_longlong, _FP, C.c_int]),
+    "dcx_set_timing_detail": (None, [_DD, C.c_int]),
+    "dcx_last_kernel_ms": (C.c_double, [_DD]),
+    "dcx_pack_boxes": (None, [_DD, _FP, C.c_int, C.POINTER(C.c_size_t),
+                              C.POINTER(C.c_size_t), _FP]),
+    "dcx_unpack_boxes": (None, [_DD, _FP, C.c_int, C.POINTER(C.c_size_t),
+                                C.POINTER(C.c_size_t), _FP]),
+    "dcx_scratch": (_FP, [_DD, C.c_size_t]),
+    "dcx_program_run": (C.c_int, [C.POINTER(ProgramArgs), _FP, _FP,
+                                  C.POINTER(C.c_double)]),
+    "dcx_write_dc_file": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t,
+                                    C.c_size_t, _FP, _FP]),
+    "dcx_default_model": (None, [C.c_size_t] + [_FP] * 6),
+    "dcx_precompute": (None, [C.c_size_t] + [_FP] * 6 + [_FP * 10]),
+    "dcx_random_boundary": (None, [C.c_int] * 14 + [_FP, _FP,
+                                                    C.POINTER(C.c_uint)]),
+    "dcx_ricker": (C.c_float, [C.c_float, C.c_int, C.c_double]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libdc_hip.so (build it first with build.py or build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python distributed-cube-average_amd/build.py)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def fp(a: np.ndarray):
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(_FP)
+
+
+def sizes_of(a: np.ndarray):
+    return (a.shape[2], a.shape[1], a.shape[0])
+
+
+# ------------------------------------------------------------ model setup --
+def default_model(n: int):
+    """vpz, vsv, epsilon, delta, phi, theta of src/precomp.c:156-169."""
+    arrs = [np.empty(n, np.float32) for _ in range(6)]
+    load().dcx_default_model(n, *[fp(a) for a in arrs])
+    return arrs
+
+
+def precompute(vpz, vsv, epsilon, delta, phi, theta):
+    """The 10 TTI coefficient arrays (src/precomp.c:85-105), same shape."""
+    ins = [np.ascontiguousarray(a, np.float32) for a in
+           (vpz, vsv, epsilon, delta, phi, theta)]
+    outs = [np.empty_like(ins[0]) for _ in range(10)]
+    load().dcx_precompute(ins[0].size, *[fp(a) for a in ins],
+                          (_FP * 10)(*[fp(o) for o in outs]))
+    return outs
+
+
+def random_boundary(local_sizes, global_sizes, start, problem, bord, absorb,
+                    vpz, vsv, seed=0):
+    """src/boundary.c:7-104 in place on vpz/vsv; returns the final seed."""
+    s = C.c_uint(seed)
+    args = list(local_sizes) + list(global_sizes) + list(start) + list(problem)
+    load().dcx_random_boundary(*[int(v) for v in args], int(bord), int(absorb),
+                               fp(vpz), fp(vsv), C.byref(s))
+    return s.value
+
+
+def ricker(dt: float, it: int, fcut: float = 40.0) -> float:
+    return load().dcx_ricker(dt, it, fcut)
+
+
+def ricker_table(dt: float, n: int, fcut: float = 40.0) -> np.ndarray:
+    return np.array([ricker(dt, i, fcut) for i in range(n)], np.float32)
+
+
+# ------------------------------------------------------------ device data --
+class DeviceData:
+    """One rank's arrays on the GPU, created through dc_device_data_init.
+
+    fields: [pp, pc, qp, qc] host arrays (or None for zeros); coef: the 10
+    coefficient arrays in dc_precomp_vars order.  The host arrays are kept
+    referenced for the lifetime of the object, as the reference's caller
+    keeps its process arrays.
+    """
+
+    def __init__(self, coef: Sequence[np.ndarray], fields=None, dx=20.0,
+                 dy=20.0, dz=20.0, dt=1e-3, kernel=KERNEL_FAST):
+        lib = load()
+        self._coef = [np.ascontiguousarray(c, np.float32) for c in coef]
+        shape = self._coef[0].shape
+        if fields is None:
+            fields = [np.zeros(shape, np.float32) for _ in range(4)]
+        self._fields = [np.ascontiguousarray(f, np.float32) for f in fields]
+        for a in self._coef + self._fields:
+            if a.shape != shape:
+                raise ValueError("all arrays must have the same shape")
+        self.shape = shape
+        self.sizes = sizes_of(self._coef[0])
+        p = Process()
+        p.sizes = _SZ3(*self.sizes)
+        p.dx, p.dy, p.dz, p.dt = dx, dy, dz, dt
+        p.source_index = -1
+        p.topology = _I3(1, 1, 1)
+        p.precomp_vars = PrecompVars(*[fp(c) for c in self._coef])
+        p.pp, p.pc, p.qp, p.qc = [fp(f) for f in self._fields]
+        self._proc = p
+        self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
+        self.ptr = lib.dc_device_data_init(C.byref(p))
+        self.set_kernel(kernel)
+
+    # -- reference ABI ------------------------------------------------------
+    def propagate(self, start, end, dx=None, dy=None, dz=None, dt=None):
+        """dc_propagate over the box [start, end)."""
+        load().dc_propagate(_SZ3(*start), _SZ3(*end), _SZ3(*self.sizes),
+                            _I3(0, 0, 0), _I3(1, 1, 1), self.ptr,
+                            self.dx if dx is None else dx,
+                            self.dy if dy is None else dy,
+                            self.dz if dz is None else dz,
+                            self.dt if dt is None else dt)
+
+    def swap(self):
+        load().dc_device_swap_arrays(self.ptr)
+
+    def add_source(self, index: int, amp: float):
+        load().dc_device_add_source(self.ptr, index, amp)
+
+    def field_token(self, name: str):
+        """The device pointer the caller passes back (data->pp etc.)."""
+        return getattr(self.ptr.contents, name)
+
+    def extract_box(self, start, end, field: str) -> np.ndarray:
+        n = int(np.prod([e - s for s, e in zip(start, end)]))
+        buf = np.empty(n, np.float32)
+        load().dc_device_extract_halo_face(self.ptr, fp(buf), _SZ3(*start),
+                                           _SZ3(*end), _SZ3(*self.sizes),
+                                           self.field_token(field))
+        return buf
+
+    def insert_box(self, buf: np.ndarray, start, end, field: str):
+        load().dc_device_insert_halo_face(
+            self.ptr, fp(np.ascontiguousarray(buf, np.float32)), _SZ3(*start),
+            _SZ3(*end), _SZ3(*self.sizes), self.field_token(field))
+
+    def get_results(self):
+        """dc_device_data_get_results: returns copies of (pc, qc)."""
+        pc = np.empty(self.shape, np.float32)
+        qc = np.empty(self.shape, np.float32)
+        self._proc.pc, self._proc.qc = fp(pc), fp(qc)
+        load().dc_device_data_get_results(C.byref(self._proc), self.ptr)
+        return pc, qc
+
+    # -- dcx extensions ------------------------------------------------------
+    def set_kernel(self, kernel: int) -> int:
+        return load().dcx_set_kernel(self.ptr, kernel)
+
+    def set_tiling(self, tile_x: int, tile_y: int, z_chunk: int) -> bool:
+        return load().dcx_set_tiling(self.ptr, tile_x, tile_y, z_chunk) == 0
+
+    def synchronize(self):
+        load().dcx_synchronize(self.ptr)
+
+    def download(self, which: int) -> np.ndarray:
+        out = np.empty(self.shape, np.float32)
+        load().dcx_download(self.ptr, which, fp(out))
+        return out
+
+    def upload(self, which: int, arr: np.ndarray):
+        load().dcx_upload(self.ptr, which,
+                          fp(np.ascontiguousarray(arr, np.float32)))
+
+    def fields(self):
+        """Current [pp, pc, qp, qc] copied to host."""
+        return [self.download(k) for k in range(4)]
+
+    def step(self, src_index=-1, src_next=0.0, add_next=False):
+        load().dcx_step(self.ptr, self.dx, self.dy, self.dz, self.dt,
+                        src_index, src_next, int(add_next))
+
+    def run(self, n_steps: int, src_index=-1, src_amp=None, graph=False):
+        """n_steps of the one-rank loop; returns elapsed device ms."""
+        amp = None
+        if src_amp is not None:
+            amp = np.ascontiguousarray(src_amp, np.float32)
+            if amp.size < n_steps:
+                raise ValueError("source table shorter than n_steps")
+        return load().dcx_run(self.ptr, self.dx, self.dy, self.dz, self.dt,
+                              n_steps, src_index,
+                              None if amp is None else fp(amp), int(graph))
+
+    def set_timing_detail(self, on: bool):
+        load().dcx_set_timing_detail(self.ptr, int(on))
+
+    def last_kernel_ms(self) -> float:
+        return load().dcx_last_kernel_ms(self.ptr)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            load().dc_device_data_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def program_run(size, absorption, d, dt, time_max, topology=(1, 1, 1),
+                kernel=KERNEL_FAST, want_fields=True):
+    """The reference program on one GPU; returns (pc, qc, stats).
+
+    pc/qc are global arrays shaped (gz, gy, gx) in the output-file layout.
+    """
+    a = ProgramArgs()
+    a.size_x, a.size_y, a.size_z = size
+    a.dx = a.dy = a.dz = d
+    a.dt, a.time_max = dt, time_max
+    a.absorption = absorption
+    a.topology = _I3(*topology)
+    a.kernel = kernel
+    g = [s + 2 * absorption + 2 * STENCIL for s in size]
+    pc = qc = None
+    if want_fields:
+        pc = np.empty((g[2], g[1], g[0]), np.float32)
+        qc = np.empty_like(pc)
+    stats = (C.c_double * 3)()
+    rc = load().dcx_program_run(C.byref(a), None if pc is None else fp(pc),
+                                None if qc is None else fp(qc), stats)
+    if rc != 0:
+        raise RuntimeError(f"dcx_program_run failed ({rc})")
+    return pc, qc, {"seconds": stats[0], "iterations": int(stats[1]),
+                    "mcells_per_s": stats[2]}

@@ -1,0 +1,153 @@
+/* libdc_hip.so -- entry points beyond the reference's backend ABI.
+ *
+ * The reference backend ABI (propagate.h, device_data.h) is what the
+ * reference's MPI worker calls; everything here is this library's own API for
+ * callers that want the MI355X-specific forms: a fused whole-step launch,
+ * device-resident time loops with HIP-graph replay, on-device halo packing,
+ * FP16 storage, batched shots, and the host-side model setup the reference
+ * does in C (src/precomp.c, src/boundary.c, src/calculate_source.c).
+ *
+ * Errors in these calls, like the reference ABI, print to stderr and exit(1),
+ * except where a function is documented to return a status.
+ */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "device_data.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- kernels ---- */
+
+/* Stencil kernels selectable for dc_propagate and the fused step.
+ *  DCX_KERNEL_FAST   2.5-D z-streaming kernel (LDS-staged xy tile with a
+ *                    4-cell apron, 8-slot register plane rings, separable
+ *                    cross derivatives, fused multiply-add).  Results agree
+ *                    with the reference to ~1e-7 relative per step.
+ *  DCX_KERNEL_EXACT  one thread per cell, the reference's operation order
+ *                    with no contraction: bit-identical to the reference's
+ *                    OpenMP path (include/sample_compute.h:105-185). */
+enum { DCX_KERNEL_FAST = 0, DCX_KERNEL_EXACT = 1 };
+
+/* Select the kernel for this struct; returns the previous choice. */
+int dcx_set_kernel(dc_device_data *data, int kernel);
+
+/* Tile shape of the fast kernel: (tile_x, tile_y) one of (32,16), (32,32),
+ * (64,16); z_chunk >= 1 planes per workgroup.  Returns 0, or -1 (and changes
+ * nothing) for an unsupported shape. */
+int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
+
+/* Wait for all queued work of this struct. */
+void dcx_synchronize(dc_device_data *data);
+
+/* Copy one field (0 pp, 1 pc, 2 qp, 3 qc) of the current pointer state to or
+ * from host memory (sizes[0]*sizes[1]*sizes[2] floats, x fastest). */
+void dcx_download(dc_device_data *data, int which, float *host);
+void dcx_upload(dc_device_data *data, int which, const float *host);
+
+/* ------------------------------------------------------ fused steps ---- */
+
+/* One whole time step of one rank with no neighbours, in one launch:
+ * update every cell of [4, sizes-4)^3, then swap (pp<->pc, qp<->qc).
+ * When src_index >= 0 and add_next is nonzero, src_next is added to the new
+ * pc and qc at src_index as the launch writes them -- the same float
+ * additions dc_device_add_source would do before the next step, so a run of
+ * dcx_step calls equals the reference's loop (src/worker.c:466-511). */
+void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
+              long long src_index, float src_next, int add_next);
+
+/* n_steps of the reference's one-rank loop: before step it, add src_amp[it]
+ * to pc and qc at src_index (skipped when src_index < 0 or src_amp is NULL),
+ * update, swap.  With use_graph nonzero the steps are captured once into a
+ * HIP graph (two step variants for the two pointer parities) and replayed.
+ * Returns the elapsed device time of the steps in milliseconds, measured with
+ * HIP events on the struct's stream. */
+double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
+               unsigned int n_steps, long long src_index,
+               const float *src_amp, int use_graph);
+
+/* Average duration (ms) of the stencil launches of the last dcx_run, from
+ * HIP events recorded around each launch when timing_detail was enabled. */
+void dcx_set_timing_detail(dc_device_data *data, int enabled);
+double dcx_last_kernel_ms(const dc_device_data *data);
+
+/* ------------------------------------------------- halo boxes on device -- */
+
+/* Pack nbox boxes of the device array `from` (a field of data, laid out with
+ * the struct's sizes) into the device buffer `dev_buf`, box after box, each x
+ * fastest; box k is [starts[3k..3k+2], ends[3k..3k+2]).  One launch. */
+void dcx_pack_boxes(dc_device_data *data, float *dev_buf, int nbox,
+                    const size_t *starts, const size_t *ends,
+                    const float *from);
+
+/* The inverse: scatter dev_buf into the boxes of `to`. */
+void dcx_unpack_boxes(dc_device_data *data, const float *dev_buf, int nbox,
+                      const size_t *starts, const size_t *ends, float *to);
+
+/* Device scratch buffer owned by the struct (grown on demand, freed with the
+ * struct), for the packed halos. */
+float *dcx_scratch(dc_device_data *data, size_t n_floats);
+
+/* ---------------------------------------------- whole program, 1 GPU ---- */
+
+/* The reference program (src/main.c) on one GPU: global grid
+ * size + 2*absorption + 2*STENCIL per axis, the default TTI model with the
+ * random-velocity boundary, a Ricker source at the grid centre, and
+ * ceil(time_max/dt) steps (float division, src/main.c:132).  The grid is cut
+ * into topology[0]*topology[1]*topology[2] subdomains exactly as the
+ * reference's coordinator cuts it for that many ranks
+ * (src/coordinator.c:19-114); all of them live on the one GPU and exchange
+ * their 4-cell halos by on-device packing and unpacking each step, in the
+ * reference's step order (src/worker.c:466-511).
+ * out_pc/out_qc (may be NULL) receive the global fields in the layout of the
+ * reference's output file (src/coordinator.c:116-219: only the compute
+ * region written, the 4-cell frame zero).  Returns 0 on success.
+ * stats[0] = elapsed seconds of the time loop, stats[1] = iterations,
+ * stats[2] = Mcell-updates/s over the global compute region. */
+typedef struct {
+  size_t size_x, size_y, size_z;
+  float dx, dy, dz, dt, time_max;
+  size_t absorption;
+  int topology[3];
+  int kernel; /* DCX_KERNEL_* */
+} dcx_program_args;
+
+int dcx_program_run(const dcx_program_args *args, float *out_pc,
+                    float *out_qc, double stats[3]);
+
+/* Write the two global fields as the reference's output file: pc then qc,
+ * float32, x fastest (src/coordinator.c:116-219).  Returns 0 on success. */
+int dcx_write_dc_file(const char *path, size_t gx, size_t gy, size_t gz,
+                      const float *pc, const float *qc);
+
+/* ---------------------------------------------------- host model setup -- */
+
+/* src/precomp.c:156-169: the reference's default homogeneous model. */
+void dcx_default_model(size_t n, float *vpz, float *vsv, float *epsilon,
+                       float *delta, float *phi, float *theta);
+
+/* src/precomp.c:85-105: the ten coefficient arrays in dc_precomp_vars order
+ * (trig in double, rounded to float before the products). */
+void dcx_precompute(size_t n, const float *vpz, const float *vsv,
+                    const float *epsilon, const float *delta, const float *phi,
+                    const float *theta, float *const out[10]);
+
+/* src/boundary.c:7-104: random-velocity boundary over one local box. */
+void dcx_random_boundary(int local_sx, int local_sy, int local_sz,
+                         int global_sx, int global_sy, int global_sz,
+                         int start_x, int start_y, int start_z, int nx, int ny,
+                         int nz, int bord, int absorb, float *vpz, float *vsv,
+                         unsigned int *seed);
+
+/* src/calculate_source.c:10-17 with the cut-off frequency as a parameter
+ * (the reference uses fcut = 40; a Ricker of peak frequency f has
+ * fcut = 3f). */
+float dcx_ricker(float dt, int it, double fcut);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,190 @@
+"""HIP paths through the C ABI against the oracle and the golden fixtures.
+
+Bars: the exact kernel is bit-identical to the reference (golden fixtures and
+oracle); the fast kernel is within 1e-5 relative L2 of the FP32 oracle on
+wavefields (its error against the FP64 oracle is printed); halo packing and
+all integer/index work are bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+import program_oracle as PO
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def rel_l2(a, b):
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) /
+                 max(np.linalg.norm(b), 1e-300))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _coef(H, g):
+    return H.precompute(g["vpz"], g["vsv"], g["epsilon"], g["delta"], g["phi"],
+                        g["theta"])
+
+
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_one_propagate_call(H, kernel):
+    g = golden("golden_cell.npz")
+    say("golden_cell kernel", kernel)
+    f0 = [g[k] for k in ("pp0", "pc0", "qp0", "qc0")]
+    d = H.DeviceData(_coef(H, g), f0, float(g["dx"]), float(g["dy"]),
+                     float(g["dz"]), float(g["dt"]), kernel=kernel)
+    sz, sy, sx = f0[0].shape
+    d.propagate((4, 4, 4), (sx - 4, sy - 4, sz - 4))
+    pp, pc, qp, qc = d.fields()
+    d.close()
+    if kernel == H.KERNEL_EXACT:
+        assert np.array_equal(bits(pp), bits(g["pp1"]))
+        assert np.array_equal(bits(qp), bits(g["qp1"]))
+    else:
+        # one step from random fields: compare the update increments
+        for new, old, ref in ((pp, g["pp0"], g["pp1"]), (qp, g["qp0"], g["qp1"])):
+            assert rel_l2(new - old, ref - old) < FAST_TOL
+    assert np.array_equal(bits(pc), bits(g["pc0"]))
+    assert np.array_equal(bits(qc), bits(g["qc0"]))
+
+
+def _reference_loop(H, d, steps, src_index, dt):
+    """src/worker.c:466-511 for one rank: source, 7 region calls, swap."""
+    for it in range(steps):
+        d.add_source(src_index, H.ricker(dt, it))
+        for a, b in PO.slabs(d.sizes):
+            d.propagate(a, b)
+        d.swap()
+
+
+@pytest.mark.parametrize("kernel", [1, 0])
+@pytest.mark.parametrize("mode", ["abi", "fused", "graph"])
+def test_forty_step_loop(H, O, kernel, mode):
+    g = golden("golden_run.npz")
+    say("golden_run", kernel, mode)
+    coef = _coef(H, g)
+    dt, steps, src = float(g["dt"]), int(g["steps"]), int(g["src_index"])
+    dx = float(g["dx"])
+    d = H.DeviceData(coef, None, dx, dx, dx, dt, kernel=kernel)
+    if mode == "abi":
+        _reference_loop(H, d, steps, src, dt)
+    else:
+        d.run(steps, src, H.ricker_table(dt, steps), graph=(mode == "graph"))
+    pc, qc = d.get_results()
+    d.close()
+    if kernel == H.KERNEL_EXACT:
+        assert np.array_equal(bits(pc), bits(g["pc"]))
+        assert np.array_equal(bits(qc), bits(g["qc"]))
+    else:
+        e = rel_l2(pc, g["pc"]), rel_l2(qc, g["qc"])
+        f64 = O.run_f64([np.zeros(pc.shape)] * 4, coef, dx, dx, dx, dt, steps,
+                        src, H.ricker_table(dt, steps))
+        say(f"fast vs reference relL2 p={e[0]:.3e} q={e[1]:.3e}; vs FP64 "
+            f"p={rel_l2(pc, f64[1]):.3e} q={rel_l2(qc, f64[3]):.3e}")
+        assert max(e) < FAST_TOL
+
+
+@pytest.mark.parametrize("tiling", [(32, 16, 64), (32, 16, 5), (32, 32, 7),
+                                    (64, 16, 64)])
+def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
+    """Odd sizes (tile overhang in x, y and z) and sub-boxes of the grid."""
+    shape = (29, 41, 70)  # (sz, sy, sx)
+    w = VM.random_small(shape, seed=7)
+    rng = np.random.default_rng(11)
+    f0 = [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+    say("ragged", tiling)
+    for box in [((4, 4, 4), (66, 37, 25)), ((9, 5, 4), (40, 36, 11)),
+                ((4, 4, 20), (8, 37, 25))]:
+        d = H.DeviceData(w.coef, f0, w.dx, w.dy, w.dz, w.dt)
+        assert d.set_tiling(*tiling)
+        d.propagate(*box)
+        pp, pc, qp, qc = d.fields()
+        d.close()
+        ref = [a.copy() for a in f0]
+        O.propagate_f32(box[0], box[1], ref[0], ref[1], ref[2], ref[3], w.coef,
+                        w.dx, w.dy, w.dz, w.dt)
+        for got, want, old in ((pp, ref[0], f0[0]), (qp, ref[2], f0[2])):
+            (x0, y0, z0), (x1, y1, z1) = box
+            inside = np.zeros(shape, bool)
+            inside[z0:z1, y0:y1, x0:x1] = True
+            assert np.array_equal(bits(got[~inside]), bits(old[~inside]))
+            assert rel_l2(got[inside] - old[inside],
+                          want[inside] - old[inside]) < FAST_TOL
+
+
+def test_halo_extract_insert_bitwise(H, O, VM):
+    shape = (20, 22, 24)
+    w = VM.random_small(shape, seed=5)
+    rng = np.random.default_rng(2)
+    f0 = [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+    d = H.DeviceData(w.coef, f0)
+    for a, b in [((4, 4, 4), (8, 18, 16)), ((0, 0, 0), (24, 22, 20)),
+                 ((20, 18, 16), (24, 22, 20)), ((5, 7, 3), (6, 8, 4))]:
+        for name, k in (("pp", 0), ("qc", 3)):
+            got = d.extract_box(a, b, name)
+            want = O.extract_box(f0[k], a, b)
+            assert np.array_equal(bits(got), bits(want))
+    buf = rng.standard_normal(4 * 14 * 12).astype(np.float32)
+    d.insert_box(buf, (0, 4, 4), (4, 18, 16), "qp")
+    want = f0[2].copy()
+    O.insert_box(want, buf, (0, 4, 4), (4, 18, 16))
+    assert np.array_equal(bits(d.download(2)), bits(want))
+    d.close()
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+@pytest.mark.parametrize("ranks,topo", [(1, (1, 1, 1)), (8, (2, 2, 2))])
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_program_matches_reference_output(H, case, ranks, topo, kernel):
+    g = golden(f"golden_dc_{case}_np{ranks}.npz")
+    say("program", case, ranks, kernel)
+    size = tuple(int(v) for v in g["size"])
+    pc, qc, st = H.program_run(size, int(g["absorption"]), float(g["d"]),
+                               float(np.float32(float(g["dt"]))),
+                               float(np.float32(float(g["tmax"]))), topo,
+                               kernel)
+    assert st["iterations"] == PO.iterations(str(g["tmax"]), str(g["dt"]))
+    if kernel == H.KERNEL_EXACT:
+        assert np.array_equal(bits(pc), bits(g["pc"]))
+        assert np.array_equal(bits(qc), bits(g["qc"]))
+    else:
+        assert rel_l2(pc, g["pc"]) < FAST_TOL and rel_l2(qc, g["qc"]) < FAST_TOL
+
+
+def test_program_uneven_topology_matches_oracle(H):
+    """3x2x1 ranks: remainders on the last rank of each axis."""
+    size, ab = (13, 11, 9), 3
+    pc, qc, _ = H.program_run(size, ab, 20.0, 0.002, 0.03, (3, 2, 1), 1)
+    want = PO.run_program(size, ab, 20.0, "0.002", "0.03", (3, 2, 1))
+    assert np.array_equal(bits(pc), bits(want[0]))
+    assert np.array_equal(bits(qc), bits(want[1]))
+
+
+def test_smooth_random_200_steps(H, O, VM):
+    """The 104^3 smooth-random workload, 200 steps, fast kernel vs oracle."""
+    w = VM.smooth_random(seed=0)
+    say("smooth_random 104^3 x200")
+    amp = w.source_table()
+    d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+    d.run(w.steps, w.src_index, amp, graph=True)
+    pc, qc = d.get_results()
+    d.close()
+    zeros = [np.zeros(w.shape, np.float32)] * 4
+    ref = O.run_f32(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, w.steps,
+                    w.src_index, amp)
+    e = rel_l2(pc, ref[1]), rel_l2(qc, ref[3])
+    tr = rel_l2(pc.reshape(-1)[w.receivers], ref[1].reshape(-1)[w.receivers])
+    say(f"smooth_random relL2 p={e[0]:.3e} q={e[1]:.3e} receivers={tr:.3e}")
+    assert max(e) < FAST_TOL and tr < FAST_TOL
