@@ -1,0 +1,299 @@
+"""Benchmark: Fletcher TTI time steps of one rank's 512^3 share on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the plain-FP32 configuration the metric is quoted on): the (0,0,0)
+octant of a 1024^3 domain split 2x2x2 -- 520^3 local arrays (512^3 compute
+cells + 4-cell halo), a 32-cell random-velocity absorbing shell on its three
+outer faces, a 10-layer 1500->4500 m/s model with 5% seeded noise and random
+TTI anisotropy, a 15 Hz Ricker source; every step is one fused stencil launch
+plus on-device packing of the 7 halo boxes (3 faces, 3 edges, 1 corner) of p
+and q and unpacking them into the receive boxes (loopback: the transfer to
+neighbour GPUs is the part one GPU cannot show).
+
+Timing: W untimed steps, then exactly K steps between two stream
+synchronisations (wall clock); value = 512^3 * K / seconds / 1e6.  The
+stencil kernel's average launch time comes from HIP events around each
+launch on the library's stream (a second pass), and gives the roofline
+fraction against MI355X's 8 TB/s with 64 algorithmic bytes per cell-update
+(read pc, qc, pp, qp and ten coefficient arrays, write pp, qp).  HBM traffic
+per launch, when --traffic is given, is read from two rocprofv3 --pmc child
+runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a copy kernel of
+known size.  The CPU baseline runs the reference's own OpenMP dc_propagate
+(oracle/_ref, when built) or the oracle's C restatement on the host cores,
+on a bounded sample of the same workload: a few steps of the 520^3 arrays
+from the GPU's field state after the timed steps.
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import importlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = ("Mcell-updates/s on 1 GPU, mixed/batched vs plain FP32, "
+          "at <=1e-3 trace err")
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_FP32 = 64
+# Published single-GPU Fletcher step throughput at 512^3 (MSamples/s =
+# Mcell-updates/s): the reference's single-node baseline code on an RTX 4070.
+PUBLISHED_512 = 3870.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def octant_halo_boxes(sizes):
+    """Send and receive boxes of the (0,0,0) rank's 7 neighbours (+x,+y,+z
+    and their combinations), src/worker.c:196-228 and 679-711."""
+    S = 4
+    send, recv = [], []
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                if dx == dy == dz == 0:
+                    continue
+                a0, a1, b0, b1 = [], [], [], []
+                for d, s in zip((dx, dy, dz), sizes):
+                    if d == 1:
+                        a0.append(s - 2 * S); a1.append(s - S)
+                        b0.append(s - S); b1.append(s)
+                    else:
+                        a0.append(S); a1.append(s - S)
+                        b0.append(S); b1.append(s - S)
+                send.append((tuple(a0), tuple(a1)))
+                recv.append((tuple(b0), tuple(b1)))
+    return send, recv
+
+
+def make_workload(VM, args):
+    t = time.time()
+    w = VM.layered_octant(seed=args.seed, local=args.local)
+    log(f"[bench] workload {w.name} {w.sizes} built in {time.time() - t:.1f}s")
+    return w
+
+
+def open_device(H, w, args):
+    d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt, kernel=H.KERNEL_FAST)
+    tx, ty, zc = args.tiling
+    if not d.set_tiling(tx, ty, zc):
+        raise SystemExit(f"unsupported tiling {args.tiling}")
+    send, recv = octant_halo_boxes(w.sizes)
+    d.set_halo_loopback(send, recv)
+    return d
+
+
+def cpu_baseline(w, fields, steps, log_prefix="[bench]"):
+    """Time `steps` one-rank steps of the reference path on the host."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    S = 4
+    sx, sy, sz = w.sizes
+    box = ((S, S, S), (sx - S, sy - S, sz - S))
+    f = [np.ascontiguousarray(a).copy() for a in fields]
+    kind = "port"
+    prop = O.propagate_f32
+    try:
+        if O.ref_available():
+            O.ref()
+            kind, prop = "reference", O.ref_propagate
+    except OSError as e:
+        log(f"{log_prefix} reference backend not loadable ({e}); using the port")
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {}
+    for mode in ("default", "ftz"):
+        O.set_ftz(mode == "ftz")
+        g = [a.copy() for a in f]
+        t = time.perf_counter()
+        for _ in range(steps):
+            prop(box[0], box[1], g[0], g[1], g[2], g[3], w.coef, w.dx, w.dy,
+                 w.dz, w.dt)
+            g = [g[1], g[0], g[3], g[2]]
+        sec = time.perf_counter() - t
+        out[mode] = w.compute_cells() * steps / sec / 1e6
+        log(f"{log_prefix} cpu {kind} {mode}: {out[mode]:.2f} Mcell/s "
+            f"({steps} steps, {sec:.1f}s, {cores} threads)")
+    O.set_ftz(False)
+    return {"value": round(out["default"], 3), "unit": "Mcell-updates/s",
+            "cores": cores, "kind": kind,
+            "sample": f"{steps} steps of the full {sx}x{sy}x{sz} arrays from the "
+                      f"GPU's field state after the timed steps, default FP "
+                      f"environment (denormals live, as the reference runs)",
+            "value_ftz_daz": round(out["ftz"], 3)}
+
+
+def pmc_child(args):
+    """Run under rocprofv3 --pmc: one calibration copy, then a few steps."""
+    pkg = importlib.import_module("distributed-cube-average_amd")
+    H = pkg.dc_hip
+    VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
+    w = make_workload(VM, args)
+    d = open_device(H, w, args)
+    d.copy_field(0, 1)  # known size: n floats read, n floats written
+    d.synchronize()
+    amp = w.source_table()
+    d.run(args.steps, w.src_index, amp[:args.steps])
+    d.synchronize()
+    d.close()
+
+
+def read_counters(path_glob):
+    """{kernel_name: [per-dispatch values]} from rocprofv3 counter CSVs."""
+    import csv
+    out = {}
+    for fn in glob.glob(path_glob, recursive=True):
+        with open(fn) as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name") or row.get("KernelName") or ""
+                val = row.get("Counter_Value") or row.get("Value")
+                if val is None:
+                    continue
+                out.setdefault(name, []).append(float(val))
+    return out
+
+
+def measure_traffic(args, n_alg_bytes, n_copy_bytes):
+    """HBM bytes per stencil launch from FETCH_SIZE/WRITE_SIZE (KiB)."""
+    res = {}
+    with tempfile.TemporaryDirectory(dir=os.path.join(REPO, "gpurun_out")
+                                     if os.path.isdir(os.path.join(REPO, "gpurun_out"))
+                                     else None) as td:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(td, counter)
+            cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter,
+                   "-d", out, "-o", "pmc", "--output-format", "csv", "--",
+                   sys.executable, os.path.abspath(__file__), "--child-pmc",
+                   "--steps", "3", "--local", str(args.local), "--seed",
+                   str(args.seed), "--tiling", *map(str, args.tiling)]
+            log(f"[bench] traffic pass {counter}")
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                log(f"[bench] rocprofv3 {counter} failed rc={r.returncode}: "
+                    f"{r.stderr[-400:]}")
+                return None
+            vals = read_counters(os.path.join(out, "**", "*counter_collection.csv"))
+            sten = [v for k, vs in vals.items() if "k_fast" in k for v in vs]
+            copy = [v for k, vs in vals.items() if "k_box_copy1" in k for v in vs]
+            if not sten or not copy:
+                log(f"[bench] no {counter} rows for the kernels: {list(vals)}")
+                return None
+            # The calibration copy is the first box-copy dispatch.
+            scale = n_copy_bytes / (copy[0] * 1024.0)
+            res[counter] = float(np.mean(sten)) * 1024.0 * scale
+            res[counter + "_scale"] = scale
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--local", type=int, default=512,
+                    help="compute cells per axis of the local block")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 64))
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", action="store_true",
+                    help="measure HBM bytes per launch with rocprofv3 --pmc")
+    ap.add_argument("--child-pmc", action="store_true", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build runs one rank's "
+              f"subdomain per GPU and no multi-GPU path is part of it "
+              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    if args.child_pmc:
+        pmc_child(args)
+        return
+
+    traffic = None
+    if args.traffic:
+        n = (args.local + 8) ** 3
+        traffic = measure_traffic(args, None, 4.0 * n)
+
+    pkg = importlib.import_module("distributed-cube-average_amd")
+    H = pkg.dc_hip
+    VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
+    w = make_workload(VM, args)
+    d = open_device(H, w, args)
+    amp = w.source_table() if w.steps >= args.warmup + args.steps else \
+        H.ricker_table(w.dt, args.warmup + args.steps, w.fcut)
+
+    log(f"[bench] warmup {args.warmup} steps")
+    d.run(args.warmup, w.src_index, amp[:args.warmup])
+    d.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    d.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = d.run(args.steps, w.src_index, amp[args.warmup:args.warmup + args.steps])
+    d.synchronize()
+    sec = time.perf_counter() - t0
+    cells = w.compute_cells()
+    value = cells * args.steps / sec / 1e6
+    log(f"[bench] {value:.1f} Mcell/s wall ({sec * 1e3 / args.steps:.3f} ms/step, "
+        f"device {dev_ms / args.steps:.3f} ms/step)")
+
+    # Per-launch stencil time (events around each launch, same stream).
+    n_detail = min(50, args.steps)
+    d.set_timing_detail(True)
+    d.run(n_detail, -1, None)
+    kern_ms = d.last_kernel_ms()
+    d.set_timing_detail(False)
+    alg_bytes = BYTES_PER_CELL_FP32 * cells
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    log(f"[bench] stencil launch {kern_ms:.3f} ms -> {achieved:.0f} GB/s "
+        f"algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+
+    fields = d.fields()
+    d.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, fields, args.cpu_steps)
+
+    roof = {"bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else
+            round(traffic["FETCH_SIZE"] + traffic["WRITE_SIZE"]),
+            "kernel": "k_fast<32,16>" if tuple(args.tiling[:2]) == (32, 16)
+            else f"k_fast<{args.tiling[0]},{args.tiling[1]}>",
+            "kernel_ms": round(kern_ms, 4),
+            "algorithmic_bytes": alg_bytes}
+    if traffic is not None:
+        roof["traffic_detail"] = {k: round(v, 4) for k, v in traffic.items()}
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(value / PUBLISHED_512, 3),
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"layered_octant_{args.local}",
+                   "local_arrays": list(w.sizes),
+                   "compute_cells_per_step": cells,
+                   "form": "plain FP32, fused source, 7-box halo pack/unpack "
+                           "loopback per step",
+                   "tiling": list(args.tiling), "dt": w.dt,
+                   "device_ms_per_step": round(dev_ms / args.steps, 4)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,6 +105,12 @@ struct State {
   int timing_detail;
   double last_kernel_ms;
   hipEvent_t ev[2];
+  // Loopback halo exchange run after every fused step (dcx_set_halo_loopback):
+  // box tables [parity][pack|unpack] in device memory and the packed buffer.
+  int halo_nbox;
+  int64_t halo_total;
+  BoxCopy *halo_tab;
+  float *halo_buf;
 };
 
 constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"

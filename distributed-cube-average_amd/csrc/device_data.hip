@@ -192,6 +192,8 @@ void dc_device_data_free(dc_device_data *data) {
   DCX_CHECK(hipFree(st->coef_mem));
   if (st->scratch) DCX_CHECK(hipFree(st->scratch));
   if (st->stage) DCX_CHECK(hipFree(st->stage));
+  if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
+  if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
   DCX_CHECK(hipEventDestroy(st->ev[0]));
   DCX_CHECK(hipEventDestroy(st->ev[1]));
   if (st->own_stream) DCX_CHECK(hipStreamDestroy(st->stream));
@@ -366,12 +368,25 @@ static Box full_box(const State *st) {
              st->sizes[2] - STENCIL};
 }
 
+// Pack the registered send boxes of the new pp/qp and unpack them into the
+// receive boxes (no-op when no loopback is registered).
+static void halo_loopback(State *st) {
+  if (st->halo_nbox == 0) return;
+  const int par = st->abi.pp == st->field_mem[0] ? 0 : 1;
+  const int nb = st->halo_nbox;
+  launch_box_copy(st->halo_tab + (2 * par) * nb, nb, st->halo_total,
+                  st->halo_buf, 0, st->stream);
+  launch_box_copy(st->halo_tab + (2 * par + 1) * nb, nb, st->halo_total,
+                  st->halo_buf, 1, st->stream);
+}
+
 void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
               long long src_index, float src_next, int add_next) {
   State *st = state_of(data);
   StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
   if (add_next) set_source(st, a, src_index, src_next);
   launch_stencil(st, a);
+  halo_loopback(st);
   dc_device_swap_arrays(data);
 }
 
@@ -401,6 +416,7 @@ double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it], st->stream));
       launch_stencil(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it + 1], st->stream));
+      halo_loopback(st);
       dc_device_swap_arrays(data);
     }
   };
@@ -483,6 +499,60 @@ static void copy_boxes(State *st, float *dev_buf, int nbox, const size_t *starts
   launch_box_copy(d_boxes, nbox, total, dev_buf, dir, st->stream);
   DCX_CHECK(hipFreeAsync(d_boxes, st->stream));
   DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void dcx_set_halo_loopback(dc_device_data *data, int nbox,
+                           const size_t *send_starts, const size_t *send_ends,
+                           const size_t *recv_starts, const size_t *recv_ends) {
+  State *st = state_of(data);
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
+  if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
+  st->halo_tab = nullptr;
+  st->halo_buf = nullptr;
+  st->halo_nbox = 0;
+  st->halo_total = 0;
+  if (nbox <= 0) return;
+  // Entries: for each box, p then q; tables for both pointer parities.
+  std::vector<BoxCopy> tab((size_t)4 * 2 * nbox);
+  int64_t total = 0;
+  const int nb = 2 * nbox;
+  for (int k = 0; k < nbox; ++k) {
+    for (int f = 0; f < 2; ++f) {
+      for (int par = 0; par < 2; ++par) {
+        float *arr = st->field_mem[2 * f + par];
+        BoxCopy pb = box_of(st, send_starts + 3 * k, send_ends + 3 * k, arr);
+        BoxCopy ub = box_of(st, recv_starts + 3 * k, recv_ends + 3 * k, arr);
+        if (pb.nx != ub.nx || pb.ny != ub.ny || pb.nz != ub.nz)
+          DCX_FAIL("send and receive box %d differ in shape", k);
+        pb.off = ub.off = total;
+        tab[(size_t)(2 * par) * nb + 2 * k + f] = pb;
+        tab[(size_t)(2 * par + 1) * nb + 2 * k + f] = ub;
+      }
+      const BoxCopy &b = tab[2 * k + f];
+      total += b.nx * b.ny * b.nz;
+    }
+  }
+  DCX_CHECK(hipMalloc(&st->halo_tab, tab.size() * sizeof(BoxCopy)));
+  DCX_CHECK(hipMemcpy(st->halo_tab, tab.data(), tab.size() * sizeof(BoxCopy),
+                      hipMemcpyHostToDevice));
+  DCX_CHECK(hipMalloc(&st->halo_buf, (size_t)total * sizeof(float)));
+  st->halo_nbox = nb;
+  st->halo_total = total;
+}
+
+void dcx_copy_field(dc_device_data *data, int dst, int src) {
+  State *st = state_of(data);
+  float *d = field_index(st, dst);
+  float *s = field_index(st, src);
+  BoxCopy b{};
+  b.arr = s;
+  b.ys = st->ys;
+  b.zs = st->zs;
+  b.nx = st->sizes[0];
+  b.ny = st->sizes[1];
+  b.nz = st->sizes[2];
+  launch_box_copy1(b, d, 0, st->stream);
 }
 
 void dcx_pack_boxes(dc_device_data *data, float *dev_buf, int nbox,

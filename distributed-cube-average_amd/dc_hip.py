@@ -104,6 +104,8 @@ SIGNATURES = {
     "dcx_unpack_boxes": (None, [_DD, _FP, C.c_int, C.POINTER(C.c_size_t),
                                 C.POINTER(C.c_size_t), _FP]),
     "dcx_scratch": (_FP, [_DD, C.c_size_t]),
+    "dcx_set_halo_loopback": (None, [_DD, C.c_int] + [C.POINTER(C.c_size_t)] * 4),
+    "dcx_copy_field": (None, [_DD, C.c_int, C.c_int]),
     "dcx_program_run": (C.c_int, [C.POINTER(ProgramArgs), _FP, _FP,
                                   C.POINTER(C.c_double)]),
     "dcx_write_dc_file": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t,
@@ -294,6 +296,18 @@ class DeviceData:
         return load().dcx_run(self.ptr, self.dx, self.dy, self.dz, self.dt,
                               n_steps, src_index,
                               None if amp is None else fp(amp), int(graph))
+
+    def set_halo_loopback(self, send_boxes, recv_boxes):
+        """Boxes as ((x0,y0,z0),(x1,y1,z1)); see dcx_set_halo_loopback."""
+        n = len(send_boxes)
+        arr = [np.ascontiguousarray(np.array([b[i] for b in boxes], np.uint64)
+                                    .reshape(-1)) for boxes in
+               (send_boxes, recv_boxes) for i in (0, 1)]
+        ptrs = [a.ctypes.data_as(C.POINTER(C.c_size_t)) for a in arr]
+        load().dcx_set_halo_loopback(self.ptr, n, *ptrs)
+
+    def copy_field(self, dst: int, src: int):
+        load().dcx_copy_field(self.ptr, dst, src)
 
     def set_timing_detail(self, on: bool):
         load().dcx_set_timing_detail(self.ptr, int(on))

@@ -88,6 +88,23 @@ void dcx_pack_boxes(dc_device_data *data, float *dev_buf, int nbox,
 void dcx_unpack_boxes(dc_device_data *data, const float *dev_buf, int nbox,
                       const size_t *starts, const size_t *ends, float *to);
 
+/* Register a loopback halo exchange that every dcx_step / dcx_run step
+ * performs after its stencil launch: box k of the new pp and qp,
+ * [send_starts[3k..], send_ends[3k..]), is packed into one device buffer (one
+ * launch for all boxes and both fields) and unpacked into
+ * [recv_starts[3k..], recv_ends[3k..]) of the same arrays (a second launch).
+ * Send and receive boxes must have equal shapes.  This is the on-device half
+ * of a multi-GPU halo exchange with the transfer taken out; nbox = 0
+ * removes it. */
+void dcx_set_halo_loopback(dc_device_data *data, int nbox,
+                           const size_t *send_starts, const size_t *send_ends,
+                           const size_t *recv_starts, const size_t *recv_ends);
+
+/* Copy field src to field dst (0 pp, 1 pc, 2 qp, 3 qc) with the library's
+ * box-copy kernel: a streaming copy of known size, used to calibrate HBM
+ * traffic counters. */
+void dcx_copy_field(dc_device_data *data, int dst, int src);
+
 /* Device scratch buffer owned by the struct (grown on demand, freed with the
  * struct), for the packed halos. */
 float *dcx_scratch(dc_device_data *data, size_t n_floats);

@@ -16,7 +16,9 @@
  * double-precision fields; they are the accuracy oracle for error norms.
  */
 #include <math.h>
+#include <omp.h>
 #include <stddef.h>
+#include <xmmintrin.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -42,6 +44,22 @@ static const float oL11 = 0.64f, oL12 = -0.16f, oL13 = 0.03047619047619047618f,
 static const float oK0 = -2.84722222222222222222f, oK1 = 1.6f, oK2 = -0.2f,
                    oK3 = 0.02539682539682539682f,
                    oK4 = -0.00178571428571428571f;
+
+/* Set (on != 0) or clear flush-to-zero and denormals-are-zero in the MXCSR
+ * of every OpenMP thread, so a CPU timing can be taken in either FP mode.
+ * The reference runs with the default environment (denormals live).
+ * Returns the number of threads touched. */
+int orc_set_ftz(int on) {
+  int n = 0;
+#pragma omp parallel reduction(+ : n)
+  {
+    unsigned int csr = _mm_getcsr();
+    csr = on ? (csr | 0x8040u) : (csr & ~0x8040u);
+    _mm_setcsr(csr);
+    n += 1;
+  }
+  return n;
+}
 
 /* Exposed so tests can pin the constants against their closed forms. */
 void orc_weights(float out[19]) {

@@ -64,6 +64,8 @@ def lib() -> C.CDLL:
         L.orc_extract_box.argtypes = [_FP, _SZ3, _SZ3, _SZ3, _FP]
         L.orc_insert_box.argtypes = [_FP, _SZ3, _SZ3, _SZ3, _FP]
         L.orc_weights.argtypes = [_FP]
+        L.orc_set_ftz.argtypes = [C.c_int]
+        L.orc_set_ftz.restype = C.c_int
         _lib = L
     return _lib
 
@@ -89,6 +91,11 @@ def _coef_arr(coef):
 def sizes_of(a: np.ndarray):
     """(sx, sy, sz) of an array shaped (sz, sy, sx)."""
     return (a.shape[2], a.shape[1], a.shape[0])
+
+
+def set_ftz(on: bool) -> int:
+    """FTZ/DAZ on or off in every OpenMP thread (CPU timing modes)."""
+    return lib().orc_set_ftz(int(on))
 
 
 def weights() -> np.ndarray:

@@ -186,7 +186,7 @@ def main():
                     pc=data[0], qc=data[1], size=np.array(c["size"]),
                     absorption=np.array(c["a"]), d=np.float32(c["d"]),
                     dt=np.array(c["dt"]), tmax=np.array(c["tmax"]),
-                    ranks=np.array(np_), argv=np.array(" ".join(cmd[3:])))
+                    ranks=np.array(np_), argv=np.array(" ".join(cmd[4:])))
     for fn in sorted(os.listdir(HERE)):
         if fn.endswith(".npz"):
             print(fn, os.path.getsize(os.path.join(HERE, fn)))

@@ -1,0 +1,58 @@
+"""The dc_hip program's command line (src/main.c:16-76 flags)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO, golden
+
+BIN = os.path.join(REPO, "distributed-cube-average_amd", "dc_hip")
+
+
+@pytest.mark.parametrize("argv", [
+    [],
+    ["--size-x", "8", "--size-y", "8", "--size-z", "8", "--dx", "20",
+     "--dy", "20", "--dz", "20", "--dt", "0.002", "-t", "0.01", "-a", "4"],
+    ["--size-x", "8", "--size-y", "8", "--size-z", "0", "--dx", "20",
+     "--dy", "20", "--dz", "20", "--dt", "0.002", "-t", "0.01", "-a", "4",
+     "-o", "/tmp/x.dc"],
+    ["--size-x", "8", "--size-y", "8", "--size-z", "8", "--dx", "20",
+     "--dy", "20", "--dz", "20", "--dt", "0.002", "-t", "0.01", "-a", "0",
+     "-o", "/tmp/x.dc"],
+])
+def test_missing_or_zero_flag_prints_usage(argv):
+    """Like argp_usage: usage on stderr, exit status 64, no GPU touched."""
+    r = subprocess.run([BIN] + argv, capture_output=True, text=True)
+    assert r.returncode == 64
+    assert "Usage" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ranks", [1, 8])
+def test_program_writes_reference_output_file(tmp_path, ranks):
+    g = golden(f"golden_dc_a_np{ranks}.npz")
+    out = tmp_path / "a.dc"
+    args = g["argv"].item().split()
+    # keep the reference's flags, replace the output path
+    i = args.index("-o")
+    args[i + 1] = str(out)
+    r = subprocess.run([BIN, *args, "--ranks", str(ranks), "--kernel", "exact"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = np.fromfile(out, np.float32)
+    want = np.concatenate([g["pc"].ravel(), g["qc"].ravel()])
+    assert np.array_equal(data.view(np.uint32), want.view(np.uint32))
+    lines = r.stdout.strip().splitlines()
+    assert lines[0] == "rank,total_time,msamples_per_s"
+    assert lines[-1].startswith("*,")
+
+
+@pytest.mark.gpu
+def test_program_unwritable_output_exits_1(tmp_path):
+    r = subprocess.run([BIN, "--size-x", "8", "--size-y", "8", "--size-z", "8",
+                        "--dx", "20", "--dy", "20", "--dz", "20", "--dt",
+                        "0.002", "-t", "0.004", "-a", "4", "-o",
+                        str(tmp_path / "no" / "such" / "dir.dc")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
