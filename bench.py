@@ -204,7 +204,7 @@ def main():
                     help="compute cells per axis of the local block")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 64))
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", action="store_true",
                     help="measure HBM bytes per launch with rocprofv3 --pmc")
