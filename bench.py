@@ -89,6 +89,7 @@ def open_device(H, w, args):
     tx, ty, zc = args.tiling
     if not d.set_tiling(tx, ty, zc):
         raise SystemExit(f"unsupported tiling {args.tiling}")
+    d.set_xcd_remap(bool(args.xcd_remap))
     send, recv = octant_halo_boxes(w.sizes)
     d.set_halo_loopback(send, recv)
     return d
@@ -140,8 +141,10 @@ def pmc_child(args):
     VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
     w = make_workload(VM, args)
     d = open_device(H, w, args)
-    d.copy_field(0, 1)  # known size: n floats read, n floats written
+    nbytes = d.copy_field(0, 1)  # known size: nbytes read, nbytes written
     d.synchronize()
+    with open(os.environ["DC_BENCH_COPY_BYTES"], "w") as fh:
+        fh.write(str(nbytes))
     amp = w.source_table()
     d.run(args.steps, w.src_index, amp[:args.steps])
     d.synchronize()
@@ -163,7 +166,7 @@ def read_counters(path_glob):
     return out
 
 
-def measure_traffic(args, n_alg_bytes, n_copy_bytes):
+def measure_traffic(args):
     """HBM bytes per stencil launch from FETCH_SIZE/WRITE_SIZE (KiB)."""
     res = {}
     with tempfile.TemporaryDirectory(dir=os.path.join(REPO, "gpurun_out")
@@ -175,9 +178,12 @@ def measure_traffic(args, n_alg_bytes, n_copy_bytes):
                    "-d", out, "-o", "pmc", "--output-format", "csv", "--",
                    sys.executable, os.path.abspath(__file__), "--child-pmc",
                    "--steps", "3", "--local", str(args.local), "--seed",
-                   str(args.seed), "--tiling", *map(str, args.tiling)]
+                   str(args.seed), "--tiling", *map(str, args.tiling),
+                   "--xcd-remap", str(args.xcd_remap)]
             log(f"[bench] traffic pass {counter}")
-            r = subprocess.run(cmd, capture_output=True, text=True)
+            nfile = os.path.join(td, counter + ".bytes")
+            r = subprocess.run(cmd, capture_output=True, text=True,
+                               env=dict(os.environ, DC_BENCH_COPY_BYTES=nfile))
             if r.returncode != 0:
                 log(f"[bench] rocprofv3 {counter} failed rc={r.returncode}: "
                     f"{r.stderr[-400:]}")
@@ -189,6 +195,7 @@ def measure_traffic(args, n_alg_bytes, n_copy_bytes):
                 log(f"[bench] no {counter} rows for the kernels: {list(vals)}")
                 return None
             # The calibration copy is the first box-copy dispatch.
+            n_copy_bytes = float(open(nfile).read())
             scale = n_copy_bytes / (copy[0] * 1024.0)
             res[counter] = float(np.mean(sten)) * 1024.0 * scale
             res[counter + "_scale"] = scale
@@ -203,7 +210,8 @@ def main():
     ap.add_argument("--local", type=int, default=512,
                     help="compute cells per axis of the local block")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 64))
+    ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 256))
+    ap.add_argument("--xcd-remap", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", action="store_true",
@@ -222,8 +230,7 @@ def main():
 
     traffic = None
     if args.traffic:
-        n = (args.local + 8) ** 3
-        traffic = measure_traffic(args, None, 4.0 * n)
+        traffic = measure_traffic(args)
 
     pkg = importlib.import_module("distributed-cube-average_amd")
     H = pkg.dc_hip
@@ -287,7 +294,8 @@ def main():
                    "compute_cells_per_step": cells,
                    "form": "plain FP32, fused source, 7-box halo pack/unpack "
                            "loopback per step",
-                   "tiling": list(args.tiling), "dt": w.dt,
+                   "tiling": list(args.tiling), "xcd_remap": args.xcd_remap,
+                   "dt": w.dt,
                    "device_ms_per_step": round(dev_ms / args.steps, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,

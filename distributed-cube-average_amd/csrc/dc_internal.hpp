@@ -58,7 +58,9 @@ struct StepArgs {
   // cell at plane src_z, in-plane offset src_xy (x + y*ys).
   int64_t src_z, src_xy;
   float src_amp;
-  int zchunk;  // planes per workgroup (fast kernel)
+  int zchunk;     // planes per workgroup (fast kernel)
+  int xcd_remap;  // give each XCD a contiguous run of tiles (fast kernel)
+  int gx, gy;     // tiles along x and y (set by the launcher)
 };
 
 void launch_exact(const StepArgs &a, hipStream_t s);
@@ -93,15 +95,21 @@ struct State {
   bool own_stream;  // false once a driver has put the struct on its stream
   int64_t sizes[3];
   int64_t n;
+  // Device layout: rows pitched to a multiple of 32 floats (128 B) and the
+  // x = 0 element of every row at offset kPad, so the first updated cell
+  // (x = STENCIL) of every row starts a 128-byte cache line.
   int64_t ys, zs;
-  float *field_mem[4];
-  float *coef_mem;
+  int64_t alloc_floats;  // per array, including the front pad
+  float *field_alloc[4];
+  float *field_mem[4];   // field_alloc + kPad
+  float *coef_alloc;     // ten arrays of alloc_floats each
   float *scratch;
   size_t scratch_n;
   float *stage;  // device staging for the ABI halo copies
   size_t stage_n;
   int kernel;
   int tile_x, tile_y, zchunk;
+  int xcd_remap;
   int timing_detail;
   double last_kernel_ms;
   hipEvent_t ev[2];
@@ -114,6 +122,11 @@ struct State {
 };
 
 constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"
+constexpr int64_t kRowAlign = 32;  // floats per 128-byte cache line
+constexpr int64_t kPad = (kRowAlign - STENCIL % kRowAlign) % kRowAlign;
+
+// Device offset of the caller's linear index x + y*sx + z*sx*sy.
+int64_t device_index(const State *st, int64_t logical);
 
 State *state_of(dc_device_data *d);
 const State *state_of(const dc_device_data *d);

@@ -64,6 +64,7 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   a.src_xy = -1;
   a.src_amp = 0.0f;
   a.zchunk = st->zchunk;
+  a.xcd_remap = st->xcd_remap;
   return a;
 }
 
@@ -81,13 +82,38 @@ void adopt_stream(State *st, hipStream_t s) {
   st->own_stream = false;
 }
 
+int64_t device_index(const State *st, int64_t logical) {
+  const int64_t plane = st->sizes[0] * st->sizes[1];
+  const int64_t z = logical / plane, r = logical % plane;
+  const int64_t y = r / st->sizes[0], x = r % st->sizes[0];
+  return x + y * st->ys + z * st->zs;
+}
+
 static void set_source(const State *st, StepArgs &a, long long index,
                        float amp) {
   if (index < 0) return;
   if (index >= st->n) DCX_FAIL("source index %lld out of range", index);
-  a.src_z = index / st->zs;
-  a.src_xy = index % st->zs;
+  const int64_t i = device_index(st, index);
+  a.src_z = i / st->zs;
+  a.src_xy = i % st->zs;
   a.src_amp = amp;
+}
+
+// Host array (x fastest, rows of sx) <-> pitched device array.
+static void copy_in(const State *st, float *dev, const float *host) {
+  DCX_CHECK(hipMemcpy2DAsync(dev, st->ys * sizeof(float), host,
+                             st->sizes[0] * sizeof(float),
+                             st->sizes[0] * sizeof(float),
+                             st->sizes[1] * st->sizes[2],
+                             hipMemcpyHostToDevice, st->stream));
+}
+
+static void copy_out(const State *st, float *host, const float *dev) {
+  DCX_CHECK(hipMemcpy2DAsync(host, st->sizes[0] * sizeof(float), dev,
+                             st->ys * sizeof(float),
+                             st->sizes[0] * sizeof(float),
+                             st->sizes[1] * st->sizes[2],
+                             hipMemcpyDeviceToHost, st->stream));
 }
 
 static float *ensure(float *&buf, size_t &cap, size_t n) {
@@ -140,34 +166,34 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   st->n = st->sizes[0] * st->sizes[1] * st->sizes[2];
   if (st->sizes[0] * st->sizes[1] >= (int64_t(1) << 31))
     DCX_FAIL("one xy plane must hold fewer than 2^31 cells");
-  st->ys = st->sizes[0];
-  st->zs = st->sizes[0] * st->sizes[1];
+  st->ys = (st->sizes[0] + kRowAlign - 1) / kRowAlign * kRowAlign;
+  st->zs = st->ys * st->sizes[1];
+  st->alloc_floats = (kPad + st->zs * st->sizes[2] + 63) / 64 * 64;
   st->kernel = DCX_KERNEL_FAST;
   st->tile_x = 32;
-  st->tile_y = 32;
-  st->zchunk = 64;
-  const size_t bytes = (size_t)st->n * sizeof(float);
+  st->tile_y = 16;
+  st->zchunk = 256;
+  st->xcd_remap = 0;
+  const size_t abytes = (size_t)st->alloc_floats * sizeof(float);
 
   float *host_fields[4] = {process->pp, process->pc, process->qp, process->qc};
   for (int k = 0; k < 4; ++k) {
-    DCX_CHECK(hipMalloc(&st->field_mem[k], bytes));
-    if (host_fields[k])
-      DCX_CHECK(hipMemcpyAsync(st->field_mem[k], host_fields[k], bytes,
-                               hipMemcpyHostToDevice, st->stream));
-    else
-      DCX_CHECK(hipMemsetAsync(st->field_mem[k], 0, bytes, st->stream));
+    DCX_CHECK(hipMalloc(&st->field_alloc[k], abytes));
+    DCX_CHECK(hipMemsetAsync(st->field_alloc[k], 0, abytes, st->stream));
+    st->field_mem[k] = st->field_alloc[k] + kPad;
+    if (host_fields[k]) copy_in(st, st->field_mem[k], host_fields[k]);
   }
   const dc_precomp_vars &pv = process->precomp_vars;
   const float *hc[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
                             pv.ch1dyz, pv.ch1dxz, pv.v2px,   pv.v2pz,
                             pv.v2sz,   pv.v2pn};
-  DCX_CHECK(hipMalloc(&st->coef_mem, bytes * NCOEF));
+  DCX_CHECK(hipMalloc(&st->coef_alloc, abytes * NCOEF));
+  DCX_CHECK(hipMemsetAsync(st->coef_alloc, 0, abytes * NCOEF, st->stream));
   float *dc[NCOEF];
   for (int k = 0; k < NCOEF; ++k) {
     if (hc[k] == nullptr) DCX_FAIL("precomp_vars array %d is NULL", k);
-    dc[k] = st->coef_mem + (size_t)k * st->n;
-    DCX_CHECK(hipMemcpyAsync(dc[k], hc[k], bytes, hipMemcpyHostToDevice,
-                             st->stream));
+    dc[k] = st->coef_alloc + (size_t)k * st->alloc_floats + kPad;
+    copy_in(st, dc[k], hc[k]);
   }
   DCX_CHECK(hipStreamSynchronize(st->stream));
 
@@ -188,8 +214,8 @@ void dc_device_data_free(dc_device_data *data) {
   State *st = state_of(data);
   DCX_CHECK(hipSetDevice(st->device));
   DCX_CHECK(hipStreamSynchronize(st->stream));
-  for (int k = 0; k < 4; ++k) DCX_CHECK(hipFree(st->field_mem[k]));
-  DCX_CHECK(hipFree(st->coef_mem));
+  for (int k = 0; k < 4; ++k) DCX_CHECK(hipFree(st->field_alloc[k]));
+  DCX_CHECK(hipFree(st->coef_alloc));
   if (st->scratch) DCX_CHECK(hipFree(st->scratch));
   if (st->stage) DCX_CHECK(hipFree(st->stage));
   if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
@@ -203,13 +229,10 @@ void dc_device_data_free(dc_device_data *data) {
 
 void dc_device_data_get_results(dc_process_t *process, dc_device_data *data) {
   State *st = state_of(data);
-  const size_t bytes = (size_t)st->n * sizeof(float);
   if (process->pc == nullptr || process->qc == nullptr)
     DCX_FAIL("process->pc/qc must be host buffers of the local size");
-  DCX_CHECK(hipMemcpyAsync(process->pc, data->pc, bytes,
-                           hipMemcpyDeviceToHost, st->stream));
-  DCX_CHECK(hipMemcpyAsync(process->qc, data->qc, bytes,
-                           hipMemcpyDeviceToHost, st->stream));
+  copy_out(st, process->pc, data->pc);
+  copy_out(st, process->qc, data->qc);
   DCX_CHECK(hipStreamSynchronize(st->stream));
 }
 
@@ -226,7 +249,8 @@ void dc_device_swap_arrays(dc_device_data *data) {
 void dc_device_add_source(dc_device_data *data, size_t index, float source) {
   State *st = state_of(data);
   if ((int64_t)index >= st->n) DCX_FAIL("source index %zu out of range", index);
-  launch_add_source(data->pc, data->qc, (int64_t)index, source, st->stream);
+  launch_add_source(data->pc, data->qc, device_index(st, (int64_t)index),
+                    source, st->stream);
 }
 
 static BoxCopy box_of(const State *st, const size_t start[3],
@@ -329,6 +353,13 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk) {
   return 0;
 }
 
+int dcx_set_xcd_remap(dc_device_data *data, int enabled) {
+  State *st = state_of(data);
+  const int prev = st->xcd_remap;
+  st->xcd_remap = enabled != 0;
+  return prev;
+}
+
 void dcx_synchronize(dc_device_data *data) {
   DCX_CHECK(hipStreamSynchronize(state_of(data)->stream));
 }
@@ -345,17 +376,13 @@ static float *field_index(State *st, int which) {
 
 void dcx_download(dc_device_data *data, int which, float *host) {
   State *st = state_of(data);
-  DCX_CHECK(hipMemcpyAsync(host, field_index(st, which),
-                           (size_t)st->n * sizeof(float),
-                           hipMemcpyDeviceToHost, st->stream));
+  copy_out(st, host, field_index(st, which));
   DCX_CHECK(hipStreamSynchronize(st->stream));
 }
 
 void dcx_upload(dc_device_data *data, int which, const float *host) {
   State *st = state_of(data);
-  DCX_CHECK(hipMemcpyAsync(field_index(st, which), host,
-                           (size_t)st->n * sizeof(float),
-                           hipMemcpyHostToDevice, st->stream));
+  copy_in(st, field_index(st, which), host);
   DCX_CHECK(hipStreamSynchronize(st->stream));
 }
 
@@ -408,8 +435,8 @@ double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
   // step it+1 is added as they are written (see dcx_step).
   auto enqueue = [&](std::vector<hipEvent_t> *evs) {
     if (src && n_steps > 0)
-      launch_add_source(st->abi.pc, st->abi.qc, src_index, src_amp[0],
-                        st->stream);
+      launch_add_source(st->abi.pc, st->abi.qc, device_index(st, src_index),
+                        src_amp[0], st->stream);
     for (unsigned it = 0; it < n_steps; ++it) {
       StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
       if (src && it + 1 < n_steps) set_source(st, a, src_index, src_amp[it + 1]);
@@ -541,18 +568,20 @@ void dcx_set_halo_loopback(dc_device_data *data, int nbox,
   st->halo_total = total;
 }
 
-void dcx_copy_field(dc_device_data *data, int dst, int src) {
+size_t dcx_copy_field(dc_device_data *data, int dst, int src) {
   State *st = state_of(data);
-  float *d = field_index(st, dst);
-  float *s = field_index(st, src);
+  const int d = dst == 0 ? 0 : dst == 1 ? 1 : dst == 2 ? 2 : 3;
+  const int s = src == 0 ? 0 : src == 1 ? 1 : src == 2 ? 2 : 3;
+  if (d != dst || s != src || d == s) DCX_FAIL("bad field pair %d <- %d", dst, src);
+  // Whole allocations, contiguous and 256-byte aligned: every byte of both
+  // is streamed exactly once.
   BoxCopy b{};
-  b.arr = s;
-  b.ys = st->ys;
-  b.zs = st->zs;
-  b.nx = st->sizes[0];
-  b.ny = st->sizes[1];
-  b.nz = st->sizes[2];
-  launch_box_copy1(b, d, 0, st->stream);
+  b.arr = st->field_alloc[s];
+  b.nx = st->alloc_floats;
+  b.ny = 1;
+  b.nz = 1;
+  launch_box_copy1(b, st->field_alloc[d], 0, st->stream);
+  return (size_t)st->alloc_floats * sizeof(float);
 }
 
 void dcx_pack_boxes(dc_device_data *data, float *dev_buf, int nbox,

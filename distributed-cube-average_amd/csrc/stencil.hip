@@ -197,8 +197,15 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   const int tx = tid % TX, ty = tid / TX;
   const int64_t zs = a.zs;
   const int ys = (int)a.ys;
-  const int tx0 = (int)a.box.x0 + (int)blockIdx.x * TX;
-  const int ty0 = (int)a.box.y0 + (int)blockIdx.y * TY;
+  // Workgroups are dealt to the 8 XCDs round-robin by linear id; remapping
+  // gives each XCD (and its L2) a contiguous run of tiles, x fastest, so the
+  // apron rows and columns a tile shares with its neighbours stay in one L2.
+  int b = blockIdx.x;
+  const int G = gridDim.x;
+  if (a.xcd_remap && G % 8 == 0) b = (b % 8) * (G / 8) + b / 8;
+  const int bx = b % a.gx, by = (b / a.gx) % a.gy, bz = b / (a.gx * a.gy);
+  const int tx0 = (int)a.box.x0 + bx * TX;
+  const int ty0 = (int)a.box.y0 + by * TY;
   // Last coordinate any updated cell reads; loads are clamped to it so that a
   // tile overhanging the box never leaves the array.
   const int xl = (int)a.box.x1 + 3, yl = (int)a.box.y1 + 3;
@@ -256,7 +263,7 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   // Byte offset of plane k: every array is smaller than 4 GiB (checked at
   // launch), so one 32-bit offset per plane serves all eleven arrays.
   const uint32_t zsb = (uint32_t)zs * 4u;
-  const int zb = (int)a.box.z0 + (int)blockIdx.z * a.zchunk;
+  const int zb = (int)a.box.z0 + bz * a.zchunk;
   const int ze = zb + a.zchunk < (int)a.box.z1 ? zb + a.zchunk : (int)a.box.z1;
 
   // Register rings (slot = plane index - (zb-4), mod 8 / mod 4).
@@ -415,13 +422,16 @@ void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
     DCX_FAIL("fast kernel needs arrays below 4 GiB (%lld planes of %lld cells)",
              (long long)(a.box.z1 + 4), (long long)a.zs);
   const int64_t gz = (nz + a.zchunk - 1) / a.zchunk;
-  if (gz > 65535) DCX_FAIL("z chunk too small for %lld planes", (long long)nz);
-  dim3 grid((unsigned)((nx + tile_x - 1) / tile_x),
-            (unsigned)((ny + tile_y - 1) / tile_y), (unsigned)gz);
+  StepArgs b = a;
+  b.gx = (int)((nx + tile_x - 1) / tile_x);
+  b.gy = (int)((ny + tile_y - 1) / tile_y);
+  const int64_t nblocks = (int64_t)b.gx * b.gy * gz;
+  if (nblocks >= (int64_t(1) << 31)) DCX_FAIL("too many tiles");
+  dim3 grid((unsigned)nblocks);
   dim3 block(tile_x * tile_y);
 #define DCX_LAUNCH(TX, TY)                                                     \
   if (tile_x == TX && tile_y == TY) {                                          \
-    hipLaunchKernelGGL((k_fast<TX, TY>), grid, block, 0, s, a);                \
+    hipLaunchKernelGGL((k_fast<TX, TY>), grid, block, 0, s, b);                \
     DCX_CHECK(hipGetLastError());                                              \
     return;                                                                    \
   }

@@ -90,6 +90,7 @@ SIGNATURES = {
     # include/dc_hip.h
     "dcx_set_kernel": (C.c_int, [_DD, C.c_int]),
     "dcx_set_tiling": (C.c_int, [_DD, C.c_int, C.c_int, C.c_int]),
+    "dcx_set_xcd_remap": (C.c_int, [_DD, C.c_int]),
     "dcx_synchronize": (None, [_DD]),
     "dcx_download": (None, [_DD, C.c_int, _FP]),
     "dcx_upload": (None, [_DD, C.c_int, _FP]),
@@ -105,7 +106,7 @@ SIGNATURES = {
                                 C.POINTER(C.c_size_t), _FP]),
     "dcx_scratch": (_FP, [_DD, C.c_size_t]),
     "dcx_set_halo_loopback": (None, [_DD, C.c_int] + [C.POINTER(C.c_size_t)] * 4),
-    "dcx_copy_field": (None, [_DD, C.c_int, C.c_int]),
+    "dcx_copy_field": (C.c_size_t, [_DD, C.c_int, C.c_int]),
     "dcx_program_run": (C.c_int, [C.POINTER(ProgramArgs), _FP, _FP,
                                   C.POINTER(C.c_double)]),
     "dcx_write_dc_file": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t,
@@ -266,6 +267,9 @@ class DeviceData:
     def set_tiling(self, tile_x: int, tile_y: int, z_chunk: int) -> bool:
         return load().dcx_set_tiling(self.ptr, tile_x, tile_y, z_chunk) == 0
 
+    def set_xcd_remap(self, on: bool) -> bool:
+        return bool(load().dcx_set_xcd_remap(self.ptr, int(on)))
+
     def synchronize(self):
         load().dcx_synchronize(self.ptr)
 
@@ -306,8 +310,9 @@ class DeviceData:
         ptrs = [a.ctypes.data_as(C.POINTER(C.c_size_t)) for a in arr]
         load().dcx_set_halo_loopback(self.ptr, n, *ptrs)
 
-    def copy_field(self, dst: int, src: int):
-        load().dcx_copy_field(self.ptr, dst, src)
+    def copy_field(self, dst: int, src: int) -> int:
+        """Calibration copy of a whole allocation; returns bytes read."""
+        return load().dcx_copy_field(self.ptr, dst, src)
 
     def set_timing_detail(self, on: bool):
         load().dcx_set_timing_detail(self.ptr, int(on))

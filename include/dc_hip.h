@@ -41,6 +41,10 @@ int dcx_set_kernel(dc_device_data *data, int kernel);
  * nothing) for an unsupported shape. */
 int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
 
+/* Fast kernel: deal tiles to the GPU's 8 XCDs in contiguous runs so that
+ * neighbouring tiles share an L2 (default on).  Returns the previous value. */
+int dcx_set_xcd_remap(dc_device_data *data, int enabled);
+
 /* Wait for all queued work of this struct. */
 void dcx_synchronize(dc_device_data *data);
 
@@ -100,10 +104,11 @@ void dcx_set_halo_loopback(dc_device_data *data, int nbox,
                            const size_t *send_starts, const size_t *send_ends,
                            const size_t *recv_starts, const size_t *recv_ends);
 
-/* Copy field src to field dst (0 pp, 1 pc, 2 qp, 3 qc) with the library's
- * box-copy kernel: a streaming copy of known size, used to calibrate HBM
- * traffic counters. */
-void dcx_copy_field(dc_device_data *data, int dst, int src);
+/* Copy the whole device allocation of field src over that of field dst
+ * (0 pp, 1 pc, 2 qp, 3 qc; padding included) with the library's box-copy
+ * kernel: a contiguous, aligned streaming copy of known size, used to
+ * calibrate HBM traffic counters.  Returns the bytes read (= bytes written). */
+size_t dcx_copy_field(dc_device_data *data, int dst, int src);
 
 /* Device scratch buffer owned by the struct (grown on demand, freed with the
  * struct), for the packed halos. */

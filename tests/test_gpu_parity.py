@@ -184,7 +184,15 @@ def test_smooth_random_200_steps(H, O, VM):
     zeros = [np.zeros(w.shape, np.float32)] * 4
     ref = O.run_f32(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, w.steps,
                     w.src_index, amp)
+    f64 = O.run_f64(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, w.steps,
+                    w.src_index, amp)
+    rec = w.receivers
     e = rel_l2(pc, ref[1]), rel_l2(qc, ref[3])
-    tr = rel_l2(pc.reshape(-1)[w.receivers], ref[1].reshape(-1)[w.receivers])
-    say(f"smooth_random relL2 p={e[0]:.3e} q={e[1]:.3e} receivers={tr:.3e}")
+    tr = rel_l2(pc.reshape(-1)[rec], ref[1].reshape(-1)[rec])
+    say(f"smooth_random relL2 vs reference FP32: p={e[0]:.3e} q={e[1]:.3e} "
+        f"receivers={tr:.3e}")
+    say(f"  vs FP64: fast p={rel_l2(pc, f64[1]):.3e} "
+        f"receivers={rel_l2(pc.reshape(-1)[rec], f64[1].reshape(-1)[rec]):.3e}; "
+        f"reference FP32 p={rel_l2(ref[1], f64[1]):.3e} receivers="
+        f"{rel_l2(ref[1].reshape(-1)[rec], f64[1].reshape(-1)[rec]):.3e}")
     assert max(e) < FAST_TOL and tr < FAST_TOL
