@@ -90,6 +90,7 @@ def open_device(H, w, args):
     if not d.set_tiling(tx, ty, zc):
         raise SystemExit(f"unsupported tiling {args.tiling}")
     d.set_xcd_remap(bool(args.xcd_remap))
+    d.set_prefetch_depth(args.prefetch)
     send, recv = octant_halo_boxes(w.sizes)
     d.set_halo_loopback(send, recv)
     return d
@@ -179,7 +180,8 @@ def measure_traffic(args):
                    sys.executable, os.path.abspath(__file__), "--child-pmc",
                    "--steps", "3", "--local", str(args.local), "--seed",
                    str(args.seed), "--tiling", *map(str, args.tiling),
-                   "--xcd-remap", str(args.xcd_remap)]
+                   "--xcd-remap", str(args.xcd_remap),
+                   "--prefetch", str(args.prefetch)]
             log(f"[bench] traffic pass {counter}")
             nfile = os.path.join(td, counter + ".bytes")
             r = subprocess.run(cmd, capture_output=True, text=True,
@@ -210,8 +212,9 @@ def main():
     ap.add_argument("--local", type=int, default=512,
                     help="compute cells per axis of the local block")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 256))
+    ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 512))
     ap.add_argument("--xcd-remap", type=int, default=0)
+    ap.add_argument("--prefetch", type=int, default=2, choices=(1, 2))
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", action="store_true",
@@ -295,6 +298,7 @@ def main():
                    "form": "plain FP32, fused source, 7-box halo pack/unpack "
                            "loopback per step",
                    "tiling": list(args.tiling), "xcd_remap": args.xcd_remap,
+                   "prefetch_planes": args.prefetch,
                    "dt": w.dt,
                    "device_ms_per_step": round(dev_ms / args.steps, 4)},
         "roofline": roof,

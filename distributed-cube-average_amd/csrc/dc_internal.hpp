@@ -60,6 +60,7 @@ struct StepArgs {
   float src_amp;
   int zchunk;     // planes per workgroup (fast kernel)
   int xcd_remap;  // give each XCD a contiguous run of tiles (fast kernel)
+  int deep;       // two-plane prefetch variant (fast kernel)
   int gx, gy;     // tiles along x and y (set by the launcher)
 };
 
@@ -110,6 +111,7 @@ struct State {
   int kernel;
   int tile_x, tile_y, zchunk;
   int xcd_remap;
+  int deep;
   int timing_detail;
   double last_kernel_ms;
   hipEvent_t ev[2];

@@ -65,6 +65,7 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   a.src_amp = 0.0f;
   a.zchunk = st->zchunk;
   a.xcd_remap = st->xcd_remap;
+  a.deep = st->deep;
   return a;
 }
 
@@ -172,8 +173,9 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   st->kernel = DCX_KERNEL_FAST;
   st->tile_x = 32;
   st->tile_y = 16;
-  st->zchunk = 256;
+  st->zchunk = 512;
   st->xcd_remap = 0;
+  st->deep = 1;
   const size_t abytes = (size_t)st->alloc_floats * sizeof(float);
 
   float *host_fields[4] = {process->pp, process->pc, process->qp, process->qc};
@@ -351,6 +353,14 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk) {
   st->tile_y = tile_y;
   st->zchunk = z_chunk;
   return 0;
+}
+
+int dcx_set_prefetch_depth(dc_device_data *data, int planes) {
+  State *st = state_of(data);
+  const int prev = st->deep ? 2 : 1;
+  if (planes != 1 && planes != 2) DCX_FAIL("prefetch depth must be 1 or 2");
+  st->deep = planes == 2;
+  return prev;
 }
 
 int dcx_set_xcd_remap(dc_device_data *data, int enabled) {

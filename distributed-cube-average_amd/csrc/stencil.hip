@@ -177,7 +177,21 @@ __device__ __forceinline__ void stg(float *base, uint32_t boff, float v) {
   *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff) = v;
 }
 
-template <int TX, int TY>
+// Per-step global inputs: coefficients of the staged plane (in-plane part of
+// H1) and of the output plane, and the output plane's old pp/qp.
+struct StepLoads {
+  float cxx, cyy, cxy, czz, cxz, cyz, vpx, vpz, vsz, vpn, opp, oqp;
+};
+// Own and apron values of a staged plane.
+struct Staged {
+  float cp, cq, ap, aq;
+};
+
+// DEEP = false: the staged plane is fetched one plane ahead and the step's
+// coefficients at the top of the step.  DEEP = true: the staged plane two
+// planes ahead and the coefficients one step ahead, doubling the bytes each
+// wave keeps in flight (more VGPRs; same occupancy at 512 threads).
+template <int TX, int TY, bool DEEP>
 __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 8;       // LDS plane row pitch (4-cell apron)
@@ -274,50 +288,76 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) Sp[k] = Hp[k] = Sq[k] = Hq[k] = 0.f;
 
-  // The staged plane's own and apron values are prefetched one plane ahead
-  // (they are needed first thing in each step); the coefficients and the
-  // old pp/qp are loaded at the top of the step that uses them.
-  float cp, cq, ap = 0.f, aq = 0.f;
-  {
-    const uint32_t b = (uint32_t)(zb - 4) * zsb;
-    cp = ldg(Gpc, b + ob);
-    cq = ldg(Gqc, b + ob);
+  const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
+  auto load_staged = [&](int k) {
+    Staged v{0.f, 0.f, 0.f, 0.f};
+    const uint32_t b = (uint32_t)(k < nend ? k : nend - 1) * zsb;
+    v.cp = ldg(Gpc, b + ob);
+    v.cq = ldg(Gqc, b + ob);
     if (la >= 0) {
-      ap = ldg(Gpc, b + oab);
-      aq = ldg(Gqc, b + oab);
+      v.ap = ldg(Gpc, b + oab);
+      v.aq = ldg(Gqc, b + oab);
     }
+    return v;
+  };
+  // Step k's loads: plane k's in-plane coefficients, output plane k-4
+  // (clamped into the chunk) for the rest.
+  auto load_step = [&](int k) {
+    StepLoads v;
+    const uint32_t bc = (uint32_t)(k < nend ? k : nend - 1) * zsb + ob;
+    const uint32_t bj = (uint32_t)(k - 4 > zb ? (k - 4 < ze ? k - 4 : ze - 1) : zb) * zsb + ob;
+    v.cxx = ldg(Cxx, bc);
+    v.cyy = ldg(Cyy, bc);
+    v.cxy = ldg(Cxy, bc);
+    v.czz = ldg(Czz, bj);
+    v.cxz = ldg(Cxz, bj);
+    v.cyz = ldg(Cyz, bj);
+    v.vpx = ldg(Vpx, bj);
+    v.vpz = ldg(Vpz, bj);
+    v.vsz = ldg(Vsz, bj);
+    v.vpn = ldg(Vpn, bj);
+    v.opp = ldg(Gpp, bj);
+    v.oqp = ldg(Gqp, bj);
+    return v;
+  };
+
+  Staged cur = load_staged(zb - 4);
+  Staged nx1 = cur;
+  StepLoads lcur{};
+  if (DEEP) {
+    nx1 = load_staged(zb - 3);
+    lcur = load_step(zb - 4);
   }
 
-  const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
   for (int nb = zb - 4; nb < nend; nb += 8) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int n = nb + u;
       if (n >= nend) break;
       // Stage plane n.
-      sP[lo] = cp;
-      sQ[lo] = cq;
+      sP[lo] = cur.cp;
+      sQ[lo] = cur.cq;
       if (la >= 0) {
-        sP[la] = ap;
-        sQ[la] = aq;
+        sP[la] = cur.ap;
+        sQ[la] = cur.aq;
       }
       __syncthreads();
 
-      // Loads of this step (plane n coefficients, output plane j = n-4
-      // clamped into the chunk) and prefetch of plane n+1.
-      const uint32_t bc = (uint32_t)n * zsb + ob;
-      const uint32_t bj = (uint32_t)(n - 4 > zb ? n - 4 : zb) * zsb + ob;
-      const uint32_t bn = (uint32_t)(n + 1 < nend ? n + 1 : nend - 1) * zsb;
-      const float cxx = ldg(Cxx, bc), cyy = ldg(Cyy, bc), cxy = ldg(Cxy, bc);
-      const float czz = ldg(Czz, bj), cxz = ldg(Cxz, bj), cyz = ldg(Cyz, bj),
-                  vpx = ldg(Vpx, bj), vpz = ldg(Vpz, bj), vsz = ldg(Vsz, bj),
-                  vpn = ldg(Vpn, bj), opp = ldg(Gpp, bj), oqp = ldg(Gqp, bj);
-      const float ncp = ldg(Gpc, bn + ob), ncq = ldg(Gqc, bn + ob);
-      float nap = 0.f, naq = 0.f;
-      if (la >= 0) {
-        nap = ldg(Gpc, bn + oab);
-        naq = ldg(Gqc, bn + oab);
+      StepLoads L;
+      Staged nxt;
+      if (DEEP) {
+        L = lcur;
+        nxt = nx1;
+        nx1 = load_staged(n + 2);
+        lcur = load_step(n + 1);
+      } else {
+        L = load_step(n);
+        nxt = load_staged(n + 1);
       }
+      const float cp = cur.cp, cq = cur.cq;
+      const float cxx = L.cxx, cyy = L.cyy, cxy = L.cxy, czz = L.czz,
+                  cxz = L.cxz, cyz = L.cyz, vpx = L.vpx, vpz = L.vpz,
+                  vsz = L.vsz, vpn = L.vpn, opp = L.opp, oqp = L.oqp;
 
       // In-plane derivatives of plane n.
       const Pairs pxq = lds_pairs(sP, lo, 1), pyq = lds_pairs(sP, lo, PW);
@@ -399,10 +439,7 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
       Sq[u & 3] = qxx + qyy;
       Hq[u & 3] = cxx * qxx + cyy * qyy + cxy * qxy;
 
-      cp = ncp;
-      cq = ncq;
-      ap = nap;
-      aq = naq;
+      cur = nxt;
     }
   }
 }
@@ -431,7 +468,10 @@ void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
   dim3 block(tile_x * tile_y);
 #define DCX_LAUNCH(TX, TY)                                                     \
   if (tile_x == TX && tile_y == TY) {                                          \
-    hipLaunchKernelGGL((k_fast<TX, TY>), grid, block, 0, s, b);                \
+    if (a.deep)                                                                \
+      hipLaunchKernelGGL((k_fast<TX, TY, true>), grid, block, 0, s, b);        \
+    else                                                                       \
+      hipLaunchKernelGGL((k_fast<TX, TY, false>), grid, block, 0, s, b);       \
     DCX_CHECK(hipGetLastError());                                              \
     return;                                                                    \
   }

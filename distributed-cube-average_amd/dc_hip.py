@@ -91,6 +91,7 @@ SIGNATURES = {
     "dcx_set_kernel": (C.c_int, [_DD, C.c_int]),
     "dcx_set_tiling": (C.c_int, [_DD, C.c_int, C.c_int, C.c_int]),
     "dcx_set_xcd_remap": (C.c_int, [_DD, C.c_int]),
+    "dcx_set_prefetch_depth": (C.c_int, [_DD, C.c_int]),
     "dcx_synchronize": (None, [_DD]),
     "dcx_download": (None, [_DD, C.c_int, _FP]),
     "dcx_upload": (None, [_DD, C.c_int, _FP]),
@@ -266,6 +267,9 @@ class DeviceData:
 
     def set_tiling(self, tile_x: int, tile_y: int, z_chunk: int) -> bool:
         return load().dcx_set_tiling(self.ptr, tile_x, tile_y, z_chunk) == 0
+
+    def set_prefetch_depth(self, planes: int) -> int:
+        return load().dcx_set_prefetch_depth(self.ptr, planes)
 
     def set_xcd_remap(self, on: bool) -> bool:
         return bool(load().dcx_set_xcd_remap(self.ptr, int(on)))

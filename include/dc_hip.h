@@ -41,6 +41,11 @@ int dcx_set_kernel(dc_device_data *data, int kernel);
  * nothing) for an unsupported shape. */
 int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
 
+/* Fast kernel prefetch depth: 1 = the staged plane one plane ahead and the
+ * coefficients at the top of their step; 2 = the staged plane two planes
+ * ahead and the coefficients one step ahead.  Returns the previous depth. */
+int dcx_set_prefetch_depth(dc_device_data *data, int planes);
+
 /* Fast kernel: deal tiles to the GPU's 8 XCDs in contiguous runs so that
  * neighbouring tiles share an L2 (default on).  Returns the previous value. */
 int dcx_set_xcd_remap(dc_device_data *data, int enabled);

@@ -96,8 +96,9 @@ def test_forty_step_loop(H, O, kernel, mode):
         assert max(e) < FAST_TOL
 
 
-@pytest.mark.parametrize("tiling", [(32, 16, 64), (32, 16, 5), (32, 32, 7),
-                                    (64, 16, 64)])
+@pytest.mark.parametrize("tiling", [(32, 16, 64, 1), (32, 16, 5, 1),
+                                    (32, 16, 64, 2), (32, 16, 3, 2),
+                                    (32, 32, 7, 1), (64, 16, 64, 2)])
 def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
     """Odd sizes (tile overhang in x, y and z) and sub-boxes of the grid."""
     shape = (29, 41, 70)  # (sz, sy, sx)
@@ -108,7 +109,8 @@ def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
     for box in [((4, 4, 4), (66, 37, 25)), ((9, 5, 4), (40, 36, 11)),
                 ((4, 4, 20), (8, 37, 25))]:
         d = H.DeviceData(w.coef, f0, w.dx, w.dy, w.dz, w.dt)
-        assert d.set_tiling(*tiling)
+        assert d.set_tiling(*tiling[:3])
+        d.set_prefetch_depth(tiling[3])
         d.propagate(*box)
         pp, pc, qp, qc = d.fields()
         d.close()
