@@ -135,6 +135,90 @@ def cpu_baseline(w, fields, steps, log_prefix="[bench]"):
             "value_ftz_daz": round(out["ftz"], 3)}
 
 
+# Storage forms measured beside the plain FP32 step: (name, field, coef).
+FORMS = [("fp32", 0, 0), ("fp32_u16coef", 0, 2), ("fp16_f32coef", 1, 0),
+         ("fp16", 1, 1), ("fp16_u16coef", 1, 2), ("bf16_u16coef", 2, 2)]
+# Half field storage scale 2^8: the field peaks near 40 (true units) at the
+# source cell and half's largest finite value is 65504.
+FIELD_SCALE_LOG2 = 8
+
+
+def time_forms(H, w, d, args, amp, cells):
+    """Mcell/s and stencil launch time of each storage form on the same rank
+    (same model, halo loopback on); the struct is converted in place."""
+    out = {}
+    base = None
+    for name, fs, cs in FORMS:
+        if not d.set_storage(fs, cs, FIELD_SCALE_LOG2):
+            continue
+        d.run(5, w.src_index, amp[:5])
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.run(args.form_steps, w.src_index, amp[:args.form_steps])
+        d.synchronize()
+        sec = time.perf_counter() - t0
+        d.set_timing_detail(True)
+        d.run(20, -1, None)
+        kms = d.last_kernel_ms()
+        d.set_timing_detail(False)
+        bpc = d.bytes_per_cell()
+        v = cells * args.form_steps / sec / 1e6
+        base = base or v
+        out[name] = {"mcell_per_s": round(v, 1), "kernel_ms": round(kms, 4),
+                     "bytes_per_cell": bpc,
+                     "hbm_frac": round(bpc * cells / (kms * 1e-3) / 1e9 /
+                                       HBM_PEAK_GBS, 4),
+                     "speedup_vs_fp32": round(v / base, 3)}
+        log(f"[bench] form {name}: {v:.0f} Mcell/s, launch {kms:.3f} ms, "
+            f"{bpc} B/cell")
+    d.set_storage(0, 0, FIELD_SCALE_LOG2)
+    return out
+
+
+def precision_study(H, VM, steps_list, log_prefix="[bench]"):
+    """Relative L2 against the FP64 oracle (CPU) of the storage forms on a
+    136^3 crop of the octant model around the source: every step's p at the
+    128x128 compute cells of the first plane below the absorbing shell
+    (traces), and the final p wavefield."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    full = VM.layered_octant(seed=0, local=512)
+    w = VM.crop(full, (208, 208, 0), 136, receiver_z=36)
+    del full
+    res = {}
+    for steps in steps_list:
+        amp = H.ricker_table(w.dt, steps, w.fcut)
+        zeros = [np.zeros(w.shape, np.float32)] * 4
+        t = time.time()
+        f64, tr64 = O.run_traces(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, steps,
+                                 w.src_index, amp, w.receivers, double=True)
+        f32, tr32 = O.run_traces(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, steps,
+                                 w.src_index, amp, w.receivers)
+        log(f"{log_prefix} precision: FP64/FP32 oracle {steps} steps "
+            f"in {time.time() - t:.0f}s")
+
+        def rel(a, b):
+            b = np.asarray(b, np.float64)
+            return float(np.linalg.norm(np.asarray(a, np.float64) - b) /
+                         np.linalg.norm(b))
+        row = {"reference_fp32_cpu": {"trace_rel_l2": rel(tr32, tr64),
+                                      "wavefield_rel_l2": rel(f32[1], f64[1])}}
+        for name, fs, cs in FORMS:
+            d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+            d.set_storage(fs, cs, FIELD_SCALE_LOG2)
+            tr, _ = d.run_traces(steps, w.src_index, amp, w.receivers)
+            pc, _ = d.get_results()
+            d.close()
+            row[name] = {"trace_rel_l2": rel(tr, tr64),
+                         "wavefield_rel_l2": rel(pc, f64[1])}
+            log(f"{log_prefix} precision {steps} steps {name}: traces "
+                f"{row[name]['trace_rel_l2']:.3e} wavefield "
+                f"{row[name]['wavefield_rel_l2']:.3e}")
+        res[str(steps)] = {k: {kk: float(f"{vv:.4g}") for kk, vv in v.items()}
+                           for k, v in row.items()}
+    return res
+
+
 def pmc_child(args):
     """Run under rocprofv3 --pmc: one calibration copy, then a few steps."""
     pkg = importlib.import_module("distributed-cube-average_amd")
@@ -217,6 +301,10 @@ def main():
     ap.add_argument("--prefetch", type=int, default=2, choices=(1, 2))
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form-steps", type=int, default=100)
+    ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--precision-study", type=int, nargs="*", default=None,
+                    help="step counts of the storage-form precision study")
     ap.add_argument("--traffic", action="store_true",
                     help="measure HBM bytes per launch with rocprofv3 --pmc")
     ap.add_argument("--child-pmc", action="store_true", help=argparse.SUPPRESS)
@@ -268,8 +356,14 @@ def main():
     log(f"[bench] stencil launch {kern_ms:.3f} ms -> {achieved:.0f} GB/s "
         f"algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
 
+    forms = None
+    if not args.no_forms:
+        forms = time_forms(H, w, d, args, amp, cells)
     fields = d.fields()
     d.close()
+    precision = None
+    if args.precision_study:
+        precision = precision_study(H, VM, args.precision_study)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, fields, args.cpu_steps)
@@ -303,6 +397,8 @@ def main():
                    "device_ms_per_step": round(dev_ms / args.steps, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "forms": forms,
+        "precision": precision,
     }
     print(json.dumps(line), flush=True)
 

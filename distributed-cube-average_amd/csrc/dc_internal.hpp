@@ -58,6 +58,18 @@ struct StepArgs {
   // cell at plane src_z, in-plane offset src_xy (x + y*ys).
   int64_t src_z, src_xy;
   float src_amp;
+  // Storage-generic views for k_fast (set by make_step_args).  Field arrays
+  // hold values in field units (true value x fscale).  Leapfrog storage:
+  // sf = pc/qc (read), of = pp/qp (read, then overwritten with the new
+  // level).  Time-difference storage: sf = P (current level, read), of = D =
+  // dscale x (P_n - P_{n-1}) (read-modify-write), nf = the new P.
+  const void *sf[2];
+  void *of[2];
+  void *nf[2];
+  const void *cf[NCOEF];
+  float c_lo[NCOEF], c_st[NCOEF];  // coefficient decode parameters
+  float dscale;                    // time-difference units per field unit
+  int fstore, cstore;              // DCX_STORE_*, DCX_COEF_*
   int zchunk;     // planes per workgroup (fast kernel)
   int xcd_remap;  // give each XCD a contiguous run of tiles (fast kernel)
   int deep;       // two-plane prefetch variant (fast kernel)
@@ -70,13 +82,15 @@ bool fast_tiling_supported(int tile_x, int tile_y);
 void launch_add_source(float *pc, float *qc, int64_t index, float amp,
                        hipStream_t s);
 
-// One box copy between a strided array and a packed buffer.
+// One box copy between a strided array and a packed buffer (elements of
+// esize bytes: 4 for FP32 fields, 2 for 16-bit storage).
 struct BoxCopy {
-  float *arr;        // strided array
+  float *arr;        // strided array (reinterpreted for esize 2)
   int64_t ys, zs;    // its strides
   int64_t x0, y0, z0;
   int64_t nx, ny, nz;
-  int64_t off;       // offset of the box in the packed buffer
+  int64_t off;       // offset of the box in the packed buffer (elements)
+  int esize;
 };
 
 // dir = 0: pack (array -> buffer), dir = 1: unpack (buffer -> array).
@@ -117,10 +131,22 @@ struct State {
   hipEvent_t ev[2];
   // Loopback halo exchange run after every fused step (dcx_set_halo_loopback):
   // box tables [parity][pack|unpack] in device memory and the packed buffer.
-  int halo_nbox;
-  int64_t halo_total;
+  int halo_nbox;       // table entries per parity and direction
+  int64_t halo_total;  // packed elements
   BoxCopy *halo_tab;
   float *halo_buf;
+  int halo_nuser;      // boxes registered by dcx_set_halo_loopback
+  size_t *halo_user;   // 12 per box: send start/end, receive start/end
+  // Compact storage (dcx_set_storage): 16-bit P (two levels) and D per
+  // field, 16-bit coefficients, and their decode parameters.
+  int fstore, cstore;
+  float fscale, dscale;
+  uint16_t *P16[2][2];  // [field][level]
+  uint16_t *D16[2];
+  int pcur;             // level of P16 holding the current P
+  uint16_t *C16;        // ten arrays of alloc_floats elements
+  float c_lo[NCOEF], c_st[NCOEF];
+  int halo_esize;       // element size the loopback tables were built for
 };
 
 constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"
@@ -139,6 +165,14 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
 
 // Launch the selected stencil kernel.
 void launch_stencil(const State *st, const StepArgs &a);
+
+// Compact storage (storage.hip).
+void storage_free(State *st);
+void storage_download(const State *st, int which, float *host);
+void storage_add_source(State *st, int64_t dev_index, float amp);
+// (Re)build the device tables of the registered loopback halo boxes for the
+// struct's current storage (device_data.hip).
+void build_halo_tables(State *st);
 
 // Move a struct onto a caller-owned stream (the struct's own stream is
 // destroyed); used by drivers that order several subdomains on one stream.

@@ -66,12 +66,40 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   a.zchunk = st->zchunk;
   a.xcd_remap = st->xcd_remap;
   a.deep = st->deep;
+  a.fstore = st->fstore;
+  a.cstore = st->cstore;
+  a.dscale = 1.0f;
+  if (st->fstore == DCX_STORE_F32) {
+    a.sf[0] = a.pc;
+    a.sf[1] = a.qc;
+    a.of[0] = a.pp;
+    a.of[1] = a.qp;
+    a.nf[0] = a.nf[1] = nullptr;
+  } else {
+    a.sf[0] = st->P16[0][st->pcur];
+    a.sf[1] = st->P16[1][st->pcur];
+    a.nf[0] = st->P16[0][1 - st->pcur];
+    a.nf[1] = st->P16[1][1 - st->pcur];
+    a.of[0] = st->D16[0];
+    a.of[1] = st->D16[1];
+    a.dscale = st->dscale;
+  }
+  for (int k = 0; k < NCOEF; ++k) {
+    a.cf[k] = st->cstore == DCX_COEF_F32
+                  ? (const void *)a.coef.c[k]
+                  : (const void *)(st->C16 + (size_t)k * st->alloc_floats);
+    a.c_lo[k] = st->c_lo[k];
+    a.c_st[k] = st->c_st[k];
+  }
   return a;
 }
 
 void launch_stencil(const State *st, const StepArgs &a) {
-  if (st->kernel == DCX_KERNEL_EXACT)
+  if (st->kernel == DCX_KERNEL_EXACT) {
+    if (st->fstore != DCX_STORE_F32 || st->cstore != DCX_COEF_F32)
+      DCX_FAIL("the exact kernel runs on FP32 storage only");
     launch_exact(a, st->stream);
+  }
   else
     launch_fast(a, st->tile_x, st->tile_y, st->stream);
 }
@@ -97,7 +125,16 @@ static void set_source(const State *st, StepArgs &a, long long index,
   const int64_t i = device_index(st, index);
   a.src_z = i / st->zs;
   a.src_xy = i % st->zs;
-  a.src_amp = amp;
+  a.src_amp = st->fstore == DCX_STORE_F32 ? amp : amp * st->fscale;
+}
+
+// Advance the struct one step: swap the leapfrog pointers, or flip the
+// current level of compact storage.
+static void advance(State *st) {
+  if (st->fstore == DCX_STORE_F32)
+    dc_device_swap_arrays(&st->abi);
+  else
+    st->pcur ^= 1;
 }
 
 // Host array (x fastest, rows of sx) <-> pitched device array.
@@ -139,6 +176,70 @@ static int select_device(const dc_process_t *p) {
   for (int i = 0; i < (int)p->num_workers && i != p->rank; i++)
     if (std::strncmp(mine, p->hostnames + len * (size_t)i, len) == 0) same++;
   return same % count;
+}
+
+static BoxCopy box_of(const State *st, const size_t start[3],
+                      const size_t end[3], float *arr) {
+  for (int k = 0; k < 3; ++k)
+    if (start[k] > end[k] || (int64_t)end[k] > st->sizes[k])
+      DCX_FAIL("box [%zu,%zu)x[%zu,%zu)x[%zu,%zu) outside the local grid",
+               start[0], end[0], start[1], end[1], start[2], end[2]);
+  BoxCopy b{};
+  b.arr = arr;
+  b.ys = st->ys;
+  b.zs = st->zs;
+  b.x0 = (int64_t)start[0];
+  b.y0 = (int64_t)start[1];
+  b.z0 = (int64_t)start[2];
+  b.nx = (int64_t)(end[0] - start[0]);
+  b.ny = (int64_t)(end[1] - start[1]);
+  b.nz = (int64_t)(end[2] - start[2]);
+  b.off = 0;
+  return b;
+}
+
+// Entries: for each registered box, p then q; tables for both parities.  The
+// arrays are those a step writes its new level to: pp/qp at the pointer
+// parity (FP32 leapfrog), or the P level after the current one (compact).
+void build_halo_tables(State *st) {
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
+  if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
+  st->halo_tab = nullptr;
+  st->halo_buf = nullptr;
+  st->halo_nbox = 0;
+  st->halo_total = 0;
+  const int nbox = st->halo_nuser;
+  if (nbox <= 0) return;
+  const bool f32 = st->fstore == DCX_STORE_F32;
+  std::vector<BoxCopy> tab((size_t)4 * 2 * nbox);
+  int64_t total = 0;
+  const int nb = 2 * nbox;
+  for (int k = 0; k < nbox; ++k) {
+    const size_t *u = st->halo_user + 12 * k;
+    for (int f = 0; f < 2; ++f) {
+      for (int par = 0; par < 2; ++par) {
+        float *arr = f32 ? st->field_mem[2 * f + par]
+                         : reinterpret_cast<float *>(st->P16[f][1 - par]);
+        BoxCopy pb = box_of(st, u, u + 3, arr);
+        BoxCopy ub = box_of(st, u + 6, u + 9, arr);
+        if (pb.nx != ub.nx || pb.ny != ub.ny || pb.nz != ub.nz)
+          DCX_FAIL("send and receive box %d differ in shape", k);
+        pb.esize = ub.esize = f32 ? 4 : 2;
+        pb.off = ub.off = total;
+        tab[(size_t)(2 * par) * nb + 2 * k + f] = pb;
+        tab[(size_t)(2 * par + 1) * nb + 2 * k + f] = ub;
+      }
+      const BoxCopy &b = tab[2 * k + f];
+      total += b.nx * b.ny * b.nz;
+    }
+  }
+  DCX_CHECK(hipMalloc(&st->halo_tab, tab.size() * sizeof(BoxCopy)));
+  DCX_CHECK(hipMemcpy(st->halo_tab, tab.data(), tab.size() * sizeof(BoxCopy),
+                      hipMemcpyHostToDevice));
+  DCX_CHECK(hipMalloc(&st->halo_buf, (size_t)total * sizeof(float)));
+  st->halo_nbox = nb;
+  st->halo_total = total;
 }
 
 }  // namespace dcx
@@ -222,6 +323,8 @@ void dc_device_data_free(dc_device_data *data) {
   if (st->stage) DCX_CHECK(hipFree(st->stage));
   if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
   if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
+  std::free(st->halo_user);
+  storage_free(st);
   DCX_CHECK(hipEventDestroy(st->ev[0]));
   DCX_CHECK(hipEventDestroy(st->ev[1]));
   if (st->own_stream) DCX_CHECK(hipStreamDestroy(st->stream));
@@ -233,6 +336,11 @@ void dc_device_data_get_results(dc_process_t *process, dc_device_data *data) {
   State *st = state_of(data);
   if (process->pc == nullptr || process->qc == nullptr)
     DCX_FAIL("process->pc/qc must be host buffers of the local size");
+  if (st->fstore != DCX_STORE_F32) {
+    storage_download(st, 1, process->pc);
+    storage_download(st, 3, process->qc);
+    return;
+  }
   copy_out(st, process->pc, data->pc);
   copy_out(st, process->qc, data->qc);
   DCX_CHECK(hipStreamSynchronize(st->stream));
@@ -251,29 +359,14 @@ void dc_device_swap_arrays(dc_device_data *data) {
 void dc_device_add_source(dc_device_data *data, size_t index, float source) {
   State *st = state_of(data);
   if ((int64_t)index >= st->n) DCX_FAIL("source index %zu out of range", index);
+  if (st->fstore != DCX_STORE_F32) {
+    storage_add_source(st, device_index(st, (int64_t)index), source);
+    return;
+  }
   launch_add_source(data->pc, data->qc, device_index(st, (int64_t)index),
                     source, st->stream);
 }
 
-static BoxCopy box_of(const State *st, const size_t start[3],
-                      const size_t end[3], float *arr) {
-  for (int k = 0; k < 3; ++k)
-    if (start[k] > end[k] || (int64_t)end[k] > st->sizes[k])
-      DCX_FAIL("box [%zu,%zu)x[%zu,%zu)x[%zu,%zu) outside the local grid",
-               start[0], end[0], start[1], end[1], start[2], end[2]);
-  BoxCopy b{};
-  b.arr = arr;
-  b.ys = st->ys;
-  b.zs = st->zs;
-  b.x0 = (int64_t)start[0];
-  b.y0 = (int64_t)start[1];
-  b.z0 = (int64_t)start[2];
-  b.nx = (int64_t)(end[0] - start[0]);
-  b.ny = (int64_t)(end[1] - start[1]);
-  b.nz = (int64_t)(end[2] - start[2]);
-  b.off = 0;
-  return b;
-}
 
 void dc_device_extract_halo_face(dc_device_data *data, float *buffer,
                                  const size_t start_coords[DIMENSIONS],
@@ -323,6 +416,8 @@ void dc_propagate(const size_t start_coords[DIMENSIONS],
   (void)topology;
   State *st = state_of(data);
   check_sizes(st, sizes);
+  if (st->fstore != DCX_STORE_F32)
+    DCX_FAIL("dc_propagate runs on FP32 field storage (dcx_set_storage)");
   for (int k = 0; k < 3; ++k)
     if (end_coords[k] <= start_coords[k]) return;
   for (int k = 0; k < 3; ++k)
@@ -386,12 +481,20 @@ static float *field_index(State *st, int which) {
 
 void dcx_download(dc_device_data *data, int which, float *host) {
   State *st = state_of(data);
+  if (st->fstore != DCX_STORE_F32) {
+    if (which < 0 || which > 3) DCX_FAIL("field index %d not in 0..3", which);
+    DCX_CHECK(hipStreamSynchronize(st->stream));
+    storage_download(st, which, host);
+    return;
+  }
   copy_out(st, host, field_index(st, which));
   DCX_CHECK(hipStreamSynchronize(st->stream));
 }
 
 void dcx_upload(dc_device_data *data, int which, const float *host) {
   State *st = state_of(data);
+  if (st->fstore != DCX_STORE_F32)
+    DCX_FAIL("dcx_upload needs FP32 field storage (dcx_set_storage)");
   copy_in(st, field_index(st, which), host);
   DCX_CHECK(hipStreamSynchronize(st->stream));
 }
@@ -409,7 +512,9 @@ static Box full_box(const State *st) {
 // receive boxes (no-op when no loopback is registered).
 static void halo_loopback(State *st) {
   if (st->halo_nbox == 0) return;
-  const int par = st->abi.pp == st->field_mem[0] ? 0 : 1;
+  const int par = st->fstore == DCX_STORE_F32
+                      ? (st->abi.pp == st->field_mem[0] ? 0 : 1)
+                      : st->pcur;
   const int nb = st->halo_nbox;
   launch_box_copy(st->halo_tab + (2 * par) * nb, nb, st->halo_total,
                   st->halo_buf, 0, st->stream);
@@ -424,7 +529,7 @@ void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
   if (add_next) set_source(st, a, src_index, src_next);
   launch_stencil(st, a);
   halo_loopback(st);
-  dc_device_swap_arrays(data);
+  advance(st);
 }
 
 void dcx_set_timing_detail(dc_device_data *data, int enabled) {
@@ -435,18 +540,64 @@ double dcx_last_kernel_ms(const dc_device_data *data) {
   return state_of(data)->last_kernel_ms;
 }
 
-double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
-               unsigned int n_steps, long long src_index, const float *src_amp,
-               int use_graph) {
-  State *st = state_of(data);
+}  // extern "C"
+
+// Receiver sampling after each step: out[step * n + r] = p at dev_idx[r]
+// (field units converted back to true values).
+template <int FS>
+__global__ void k_gather(const void *field, const int64_t *__restrict__ idx,
+                         int n, float *__restrict__ out, float inv_scale) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float v;
+  if (FS == DCX_STORE_F32) {
+    v = static_cast<const float *>(field)[idx[r]];
+  } else if (FS == DCX_STORE_F16) {
+    v = (float)__builtin_bit_cast(_Float16, static_cast<const uint16_t *>(field)[idx[r]]);
+  } else {
+    v = __uint_as_float((uint32_t)static_cast<const uint16_t *>(field)[idx[r]] << 16);
+  }
+  out[r] = v * inv_scale;
+}
+
+struct TraceCtx {
+  int n;
+  const int64_t *idx;  // device offsets
+  float *buf;          // device, n_steps * n
+};
+
+static void gather(State *st, const TraceCtx &tc, unsigned it) {
+  const void *field = st->fstore == DCX_STORE_F32
+                          ? (const void *)st->abi.pc
+                          : (const void *)st->P16[0][st->pcur];
+  const float inv = st->fstore == DCX_STORE_F32 ? 1.0f : 1.0f / st->fscale;
+  float *out = tc.buf + (size_t)it * tc.n;
+  const dim3 g((tc.n + 255) / 256), b(256);
+  if (st->fstore == DCX_STORE_F32)
+    hipLaunchKernelGGL(k_gather<DCX_STORE_F32>, g, b, 0, st->stream, field, tc.idx, tc.n, out, inv);
+  else if (st->fstore == DCX_STORE_F16)
+    hipLaunchKernelGGL(k_gather<DCX_STORE_F16>, g, b, 0, st->stream, field, tc.idx, tc.n, out, inv);
+  else
+    hipLaunchKernelGGL(k_gather<DCX_STORE_BF16>, g, b, 0, st->stream, field, tc.idx, tc.n, out, inv);
+  DCX_CHECK(hipGetLastError());
+}
+
+static double run_impl(State *st, float dx, float dy, float dz, float dt,
+                       unsigned int n_steps, long long src_index,
+                       const float *src_amp, int use_graph, const TraceCtx *tc) {
+  dc_device_data *data = &st->abi;
   const bool src = src_index >= 0 && src_amp != nullptr;
   if (src && src_index >= st->n) DCX_FAIL("source index out of range");
   // Step it writes the fields that step it+1 reads as pc/qc; the source of
   // step it+1 is added as they are written (see dcx_step).
   auto enqueue = [&](std::vector<hipEvent_t> *evs) {
-    if (src && n_steps > 0)
-      launch_add_source(st->abi.pc, st->abi.qc, device_index(st, src_index),
-                        src_amp[0], st->stream);
+    if (src && n_steps > 0) {
+      if (st->fstore == DCX_STORE_F32)
+        launch_add_source(st->abi.pc, st->abi.qc, device_index(st, src_index),
+                          src_amp[0], st->stream);
+      else
+        storage_add_source(st, device_index(st, src_index), src_amp[0]);
+    }
     for (unsigned it = 0; it < n_steps; ++it) {
       StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
       if (src && it + 1 < n_steps) set_source(st, a, src_index, src_amp[it + 1]);
@@ -454,15 +605,18 @@ double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
       launch_stencil(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it + 1], st->stream));
       halo_loopback(st);
-      dc_device_swap_arrays(data);
+      advance(st);
+      if (tc) gather(st, *tc, it);
     }
   };
   float ms = 0.f;
   if (use_graph) {
     // Capture the whole loop, then replay it once.  Pointer swaps happen on
-    // the host during capture, so restore them before the replay's effect is
-    // accounted: the captured graph already encodes them.
+    // the host during capture; the captured graph already encodes them, so
+    // the replay starts from the pre-capture state and ends in the captured
+    // final state.
     dc_device_data before = st->abi;
+    const int pcur_before = st->pcur;
     hipGraph_t g;
     hipGraphExec_t ge;
     DCX_CHECK(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
@@ -470,12 +624,15 @@ double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
     DCX_CHECK(hipStreamEndCapture(st->stream, &g));
     DCX_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     dc_device_data after = st->abi;
+    const int pcur_after = st->pcur;
     st->abi = before;
+    st->pcur = pcur_before;
     DCX_CHECK(hipEventRecord(st->ev[0], st->stream));
     DCX_CHECK(hipGraphLaunch(ge, st->stream));
     DCX_CHECK(hipEventRecord(st->ev[1], st->stream));
     DCX_CHECK(hipEventSynchronize(st->ev[1]));
     st->abi = after;
+    st->pcur = pcur_after;
     DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
     DCX_CHECK(hipGraphExecDestroy(ge));
     DCX_CHECK(hipGraphDestroy(g));
@@ -502,7 +659,50 @@ double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
     DCX_CHECK(hipEventSynchronize(st->ev[1]));
     DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
   }
+  (void)data;
   return (double)ms;
+}
+
+extern "C" {
+
+double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
+               unsigned int n_steps, long long src_index, const float *src_amp,
+               int use_graph) {
+  return run_impl(state_of(data), dx, dy, dz, dt, n_steps, src_index, src_amp,
+                  use_graph, nullptr);
+}
+
+double dcx_run_traces(dc_device_data *data, float dx, float dy, float dz,
+                      float dt, unsigned int n_steps, long long src_index,
+                      const float *src_amp, int n_rec,
+                      const long long *receivers, float *traces,
+                      int use_graph) {
+  State *st = state_of(data);
+  if (n_rec <= 0 || n_steps == 0)
+    return run_impl(st, dx, dy, dz, dt, n_steps, src_index, src_amp, use_graph,
+                    nullptr);
+  std::vector<int64_t> idx(n_rec);
+  for (int r = 0; r < n_rec; ++r) {
+    if (receivers[r] < 0 || receivers[r] >= st->n)
+      DCX_FAIL("receiver %d index %lld out of range", r, receivers[r]);
+    idx[r] = device_index(st, receivers[r]);
+  }
+  TraceCtx tc{n_rec, nullptr, nullptr};
+  int64_t *d_idx = nullptr;
+  float *d_buf = nullptr;
+  DCX_CHECK(hipMalloc(&d_idx, n_rec * sizeof(int64_t)));
+  DCX_CHECK(hipMalloc(&d_buf, (size_t)n_rec * n_steps * sizeof(float)));
+  DCX_CHECK(hipMemcpy(d_idx, idx.data(), n_rec * sizeof(int64_t),
+                      hipMemcpyHostToDevice));
+  tc.idx = d_idx;
+  tc.buf = d_buf;
+  const double ms = run_impl(st, dx, dy, dz, dt, n_steps, src_index, src_amp,
+                             use_graph, &tc);
+  DCX_CHECK(hipMemcpy(traces, d_buf, (size_t)n_rec * n_steps * sizeof(float),
+                      hipMemcpyDeviceToHost));
+  DCX_CHECK(hipFree(d_idx));
+  DCX_CHECK(hipFree(d_buf));
+  return ms;
 }
 
 float *dcx_scratch(dc_device_data *data, size_t n_floats) {
@@ -542,40 +742,21 @@ void dcx_set_halo_loopback(dc_device_data *data, int nbox,
                            const size_t *send_starts, const size_t *send_ends,
                            const size_t *recv_starts, const size_t *recv_ends) {
   State *st = state_of(data);
-  DCX_CHECK(hipStreamSynchronize(st->stream));
-  if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
-  if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
-  st->halo_tab = nullptr;
-  st->halo_buf = nullptr;
-  st->halo_nbox = 0;
-  st->halo_total = 0;
-  if (nbox <= 0) return;
-  // Entries: for each box, p then q; tables for both pointer parities.
-  std::vector<BoxCopy> tab((size_t)4 * 2 * nbox);
-  int64_t total = 0;
-  const int nb = 2 * nbox;
-  for (int k = 0; k < nbox; ++k) {
-    for (int f = 0; f < 2; ++f) {
-      for (int par = 0; par < 2; ++par) {
-        float *arr = st->field_mem[2 * f + par];
-        BoxCopy pb = box_of(st, send_starts + 3 * k, send_ends + 3 * k, arr);
-        BoxCopy ub = box_of(st, recv_starts + 3 * k, recv_ends + 3 * k, arr);
-        if (pb.nx != ub.nx || pb.ny != ub.ny || pb.nz != ub.nz)
-          DCX_FAIL("send and receive box %d differ in shape", k);
-        pb.off = ub.off = total;
-        tab[(size_t)(2 * par) * nb + 2 * k + f] = pb;
-        tab[(size_t)(2 * par + 1) * nb + 2 * k + f] = ub;
+  std::free(st->halo_user);
+  st->halo_user = nullptr;
+  st->halo_nuser = nbox > 0 ? nbox : 0;
+  if (nbox > 0) {
+    st->halo_user = static_cast<size_t *>(std::malloc(sizeof(size_t) * 12 * nbox));
+    if (!st->halo_user) DCX_FAIL("OOM");
+    for (int k = 0; k < nbox; ++k)
+      for (int a = 0; a < 3; ++a) {
+        st->halo_user[12 * k + a] = send_starts[3 * k + a];
+        st->halo_user[12 * k + 3 + a] = send_ends[3 * k + a];
+        st->halo_user[12 * k + 6 + a] = recv_starts[3 * k + a];
+        st->halo_user[12 * k + 9 + a] = recv_ends[3 * k + a];
       }
-      const BoxCopy &b = tab[2 * k + f];
-      total += b.nx * b.ny * b.nz;
-    }
   }
-  DCX_CHECK(hipMalloc(&st->halo_tab, tab.size() * sizeof(BoxCopy)));
-  DCX_CHECK(hipMemcpy(st->halo_tab, tab.data(), tab.size() * sizeof(BoxCopy),
-                      hipMemcpyHostToDevice));
-  DCX_CHECK(hipMalloc(&st->halo_buf, (size_t)total * sizeof(float)));
-  st->halo_nbox = nb;
-  st->halo_total = total;
+  build_halo_tables(st);
 }
 
 size_t dcx_copy_field(dc_device_data *data, int dst, int src) {

@@ -16,6 +16,25 @@ namespace dcx {
 // volumes are below 2^32, so in-box coordinates use 32-bit division.
 constexpr int kMaxBoxesLds = 512;
 
+// Element i of the box's array <-> element e of the packed buffer, in the
+// box's element size (the buffer uses the same element size).
+__device__ __forceinline__ void copy_elem(const BoxCopy &b, int64_t i,
+                                          float *buf, int64_t e, int dir) {
+  if (b.esize == 2) {
+    uint16_t *arr = reinterpret_cast<uint16_t *>(b.arr);
+    uint16_t *pk = reinterpret_cast<uint16_t *>(buf);
+    if (dir == 0)
+      pk[e] = arr[i];
+    else
+      arr[i] = pk[e];
+  } else {
+    if (dir == 0)
+      buf[e] = b.arr[i];
+    else
+      b.arr[i] = buf[e];
+  }
+}
+
 __global__ void __launch_bounds__(256)
     k_box_copy(const BoxCopy *__restrict__ boxes, int nbox, int64_t total,
                float *__restrict__ buf, int dir) {
@@ -40,11 +59,8 @@ __global__ void __launch_bounds__(256)
     const int64_t x = k - r * nx;
     const int64_t z = r / ny;
     const int64_t y = r - (uint32_t)z * ny;
-    float *p = b.arr + (b.x0 + x) + (b.y0 + y) * b.ys + (b.z0 + z) * b.zs;
-    if (dir == 0)
-      buf[e] = *p;
-    else
-      *p = buf[e];
+    const int64_t i = (b.x0 + x) + (b.y0 + y) * b.ys + (b.z0 + z) * b.zs;
+    copy_elem(b, i, buf, e, dir);
   }
 }
 
@@ -59,11 +75,8 @@ __global__ void __launch_bounds__(256)
     const int64_t x = k - r * nx;
     const int64_t z = r / ny;
     const int64_t y = r - (uint32_t)z * ny;
-    float *p = b.arr + (b.x0 + x) + (b.y0 + y) * b.ys + (b.z0 + z) * b.zs;
-    if (dir == 0)
-      buf[b.off + e] = *p;
-    else
-      *p = buf[b.off + e];
+    const int64_t i = (b.x0 + x) + (b.y0 + y) * b.ys + (b.z0 + z) * b.zs;
+    copy_elem(b, i, buf, b.off + e, dir);
   }
 }
 

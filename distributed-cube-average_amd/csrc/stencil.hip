@@ -169,16 +169,80 @@ __device__ __forceinline__ float d1_lds(const float *L, int l, int st) {
 // Load/store at a wave-uniform base plus a per-lane unsigned 32-bit byte
 // offset, the form gfx950's global_load/store take with an SGPR base address
 // (one VGPR per offset instead of a 64-bit address pair per load).
-__device__ __forceinline__ float ldg(const float *base, uint32_t boff) {
-  return *reinterpret_cast<const float *>(
-      reinterpret_cast<const char *>(base) + boff);
+template <class T>
+__device__ __forceinline__ T ldb(const void *base, uint32_t boff) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      boff);
 }
-__device__ __forceinline__ void stg(float *base, uint32_t boff, float v) {
-  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff) = v;
+template <class T>
+__device__ __forceinline__ void stb(void *base, uint32_t boff, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
 }
 
+// Field storage policies (DCX_STORE_*): element size, load as float, store.
+template <int FS> struct FieldIO;
+template <> struct FieldIO<DCX_STORE_F32> {
+  static constexpr uint32_t kBytes = 4;
+  static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
+    return ldb<float>(b, o);
+  }
+  static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
+    stb<float>(b, o, v);
+  }
+};
+template <> struct FieldIO<DCX_STORE_F16> {
+  static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
+    return (float)__builtin_bit_cast(_Float16, ldb<uint16_t>(b, o));
+  }
+  static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
+    stb<uint16_t>(b, o, __builtin_bit_cast(uint16_t, (_Float16)v));
+  }
+};
+template <> struct FieldIO<DCX_STORE_BF16> {
+  static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
+    return __uint_as_float((uint32_t)ldb<uint16_t>(b, o) << 16);
+  }
+  static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    stb<uint16_t>(b, o, (uint16_t)(u >> 16));
+  }
+};
+
+// Coefficient storage policies (DCX_COEF_*): element size and decode of
+// array k (a compile-time index after unrolling).
+template <int CS> struct CoefIO;
+template <> struct CoefIO<DCX_COEF_F32> {
+  static constexpr uint32_t kBytes = 4;
+  static __device__ __forceinline__ float ld(const StepArgs &a, int k,
+                                             uint32_t o) {
+    return ldb<float>(a.cf[k], o);
+  }
+};
+template <> struct CoefIO<DCX_COEF_F16> {
+  static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ float ld(const StepArgs &a, int k,
+                                             uint32_t o) {
+    return (float)__builtin_bit_cast(_Float16, ldb<uint16_t>(a.cf[k], o)) *
+           a.c_st[k];
+  }
+};
+template <> struct CoefIO<DCX_COEF_U16> {
+  static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ float ld(const StepArgs &a, int k,
+                                             uint32_t o) {
+    const uint32_t u = ldb<uint16_t>(a.cf[k], o);
+    const float t = fmaf((float)u, a.c_st[k], a.c_lo[k]);
+    if (k < V2PX) return t;              // angle terms: affine
+    return u ? exp2f(t) : 0.0f;          // v^2 terms: affine in log2
+  }
+};
+
 // Per-step global inputs: coefficients of the staged plane (in-plane part of
-// H1) and of the output plane, and the output plane's old pp/qp.
+// H1) and of the output plane, and the output plane's old state (previous
+// level for leapfrog storage, time difference otherwise).
 struct StepLoads {
   float cxx, cyy, cxy, czz, cxz, cyz, vpx, vpz, vsz, vpn, opp, oqp;
 };
@@ -191,8 +255,12 @@ struct Staged {
 // coefficients at the top of the step.  DEEP = true: the staged plane two
 // planes ahead and the coefficients one step ahead, doubling the bytes each
 // wave keeps in flight (more VGPRs; same occupancy at 512 threads).
-template <int TX, int TY, bool DEEP>
+// FS/CS select the field and coefficient storage.
+template <int TX, int TY, bool DEEP, int FS, int CS>
 __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
+  using F = FieldIO<FS>;
+  using Cf = CoefIO<CS>;
+  constexpr bool kTimeDiff = FS != DCX_STORE_F32;
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 8;       // LDS plane row pitch (4-cell apron)
   constexpr int PH = TY + 8;
@@ -227,9 +295,10 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   const int x = tx0 + tx, y = ty0 + ty;
   const bool act = x < a.box.x1 && y < a.box.y1;
   // In-plane offset of this thread's cell (one plane holds < 2^31 cells) and
-  // its byte offset.
+  // its byte offsets in field and coefficient arrays.
   const int own = (x < xl ? x : xl) + (y < yl ? y : yl) * ys;
-  const uint32_t ob = (uint32_t)own * 4u;
+  const uint32_t ob = (uint32_t)own * F::kBytes;
+  const uint32_t obc = (uint32_t)own * Cf::kBytes;
   const int lo = (ty + 4) * PW + tx + 4;  // own cell in the LDS plane
   const int ld = (ty + 4) * TX + tx;      // own cell in the D1x plane
 
@@ -256,7 +325,7 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
     gy = gy < yl ? gy : yl;
     oa = gx + gy * ys;
   }
-  const uint32_t oab = (uint32_t)oa * 4u;
+  const uint32_t oab = (uint32_t)oa * F::kBytes;
   // D1x of the apron rows: rows 0..3 and TY+4..TY+7 of the D1x plane.
   int dr = -1, dc = 0;
   if (tid < NDXA) {
@@ -266,17 +335,16 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   }
 
   const float dt2 = a.dt * a.dt;
-  // Base addresses (wave-uniform, SGPRs).
-  const float *const Cxx = a.coef.c[CXX], *const Cyy = a.coef.c[CYY],
-                     *const Cxy = a.coef.c[CXY], *const Czz = a.coef.c[CZZ],
-                     *const Cxz = a.coef.c[CXZ], *const Cyz = a.coef.c[CYZ],
-                     *const Vpx = a.coef.c[V2PX], *const Vpz = a.coef.c[V2PZ],
-                     *const Vsz = a.coef.c[V2SZ], *const Vpn = a.coef.c[V2PN];
-  const float *const Gpc = a.pc, *const Gqc = a.qc;
-  float *const Gpp = a.pp, *const Gqp = a.qp;
+  // Time-difference storage keeps D in dscale units of the field.
+  const float dt2d = dt2 * a.dscale;
+  const float inv_d = 1.0f / a.dscale;
+  const void *const Gpc = a.sf[0], *const Gqc = a.sf[1];
+  void *const Gpp = a.of[0], *const Gqp = a.of[1];
+  void *const Gpn = a.nf[0], *const Gqn = a.nf[1];
   // Byte offset of plane k: every array is smaller than 4 GiB (checked at
-  // launch), so one 32-bit offset per plane serves all eleven arrays.
-  const uint32_t zsb = (uint32_t)zs * 4u;
+  // launch), so one 32-bit offset per plane serves all arrays of a kind.
+  const uint32_t zsb = (uint32_t)zs * F::kBytes;
+  const uint32_t zsbc = (uint32_t)zs * Cf::kBytes;
   const int zb = (int)a.box.z0 + bz * a.zchunk;
   const int ze = zb + a.zchunk < (int)a.box.z1 ? zb + a.zchunk : (int)a.box.z1;
 
@@ -291,12 +359,12 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
   auto load_staged = [&](int k) {
     Staged v{0.f, 0.f, 0.f, 0.f};
-    const uint32_t b = (uint32_t)(k < nend ? k : nend - 1) * zsb;
-    v.cp = ldg(Gpc, b + ob);
-    v.cq = ldg(Gqc, b + ob);
+    const uint32_t bb = (uint32_t)(k < nend ? k : nend - 1) * zsb;
+    v.cp = F::ld(Gpc, bb + ob);
+    v.cq = F::ld(Gqc, bb + ob);
     if (la >= 0) {
-      v.ap = ldg(Gpc, b + oab);
-      v.aq = ldg(Gqc, b + oab);
+      v.ap = F::ld(Gpc, bb + oab);
+      v.aq = F::ld(Gqc, bb + oab);
     }
     return v;
   };
@@ -304,20 +372,23 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   // (clamped into the chunk) for the rest.
   auto load_step = [&](int k) {
     StepLoads v;
-    const uint32_t bc = (uint32_t)(k < nend ? k : nend - 1) * zsb + ob;
-    const uint32_t bj = (uint32_t)(k - 4 > zb ? (k - 4 < ze ? k - 4 : ze - 1) : zb) * zsb + ob;
-    v.cxx = ldg(Cxx, bc);
-    v.cyy = ldg(Cyy, bc);
-    v.cxy = ldg(Cxy, bc);
-    v.czz = ldg(Czz, bj);
-    v.cxz = ldg(Cxz, bj);
-    v.cyz = ldg(Cyz, bj);
-    v.vpx = ldg(Vpx, bj);
-    v.vpz = ldg(Vpz, bj);
-    v.vsz = ldg(Vsz, bj);
-    v.vpn = ldg(Vpn, bj);
-    v.opp = ldg(Gpp, bj);
-    v.oqp = ldg(Gqp, bj);
+    const int kc = k < nend ? k : nend - 1;
+    const int kj = k - 4 > zb ? (k - 4 < ze ? k - 4 : ze - 1) : zb;
+    const uint32_t bc = (uint32_t)kc * zsbc + obc;
+    const uint32_t bjc = (uint32_t)kj * zsbc + obc;
+    const uint32_t bjf = (uint32_t)kj * zsb + ob;
+    v.cxx = Cf::ld(a, CXX, bc);
+    v.cyy = Cf::ld(a, CYY, bc);
+    v.cxy = Cf::ld(a, CXY, bc);
+    v.czz = Cf::ld(a, CZZ, bjc);
+    v.cxz = Cf::ld(a, CXZ, bjc);
+    v.cyz = Cf::ld(a, CYZ, bjc);
+    v.vpx = Cf::ld(a, V2PX, bjc);
+    v.vpz = Cf::ld(a, V2PZ, bjc);
+    v.vsz = Cf::ld(a, V2SZ, bjc);
+    v.vpn = Cf::ld(a, V2PN, bjc);
+    v.opp = F::ld(Gpp, bjf);
+    v.oqp = F::ld(Gqp, bjf);
     return v;
   };
 
@@ -414,17 +485,37 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
         const float h2q = Sq[D] + qzz - h1q;
         const float rhsp = vpx * h2p + vpz * h1q + vsz * (h1p - h1q);
         const float rhsq = vpn * h2p + vpz * h1q - vsz * (h2p - h2q);
-        float pn = 2.0f * rp[J] - opp + rhsp * dt2;
-        float qn = 2.0f * rq[J] - oqp + rhsq * dt2;
         const int j = n - 4;
-        if (j == a.src_z && (int64_t)own == a.src_xy) {
-          pn += a.src_amp;
-          qn += a.src_amp;
-        }
-        if (act) {
-          const uint32_t bo = (uint32_t)j * zsb + ob;
-          stg(Gpp, bo, pn);
-          stg(Gqp, bo, qn);
+        const bool src = j == a.src_z && (int64_t)own == a.src_xy;
+        const uint32_t bo = (uint32_t)j * zsb + ob;
+        if (kTimeDiff) {
+          // D_new = D + dt^2 rhs; P_new = P + D_new; the next step's source
+          // raises P_new, and with it D_new, by src_amp (field units).
+          float dnp = opp + rhsp * dt2d, dnq = oqp + rhsq * dt2d;
+          float pn = rp[J] + dnp * inv_d, qn = rq[J] + dnq * inv_d;
+          if (src) {
+            pn += a.src_amp;
+            qn += a.src_amp;
+            dnp += a.src_amp * a.dscale;
+            dnq += a.src_amp * a.dscale;
+          }
+          if (act) {
+            F::st(Gpn, bo, pn);
+            F::st(Gqn, bo, qn);
+            F::st(Gpp, bo, dnp);
+            F::st(Gqp, bo, dnq);
+          }
+        } else {
+          float pn = 2.0f * rp[J] - opp + rhsp * dt2;
+          float qn = 2.0f * rq[J] - oqp + rhsq * dt2;
+          if (src) {
+            pn += a.src_amp;
+            qn += a.src_amp;
+          }
+          if (act) {
+            F::st(Gpp, bo, pn);
+            F::st(Gqp, bo, qn);
+          }
         }
       }
       // Push plane n into the rings.
@@ -449,6 +540,33 @@ bool fast_tiling_supported(int tx, int ty) {
   return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && ty == 16);
 }
 
+template <int TX, int TY, int FS, int CS>
+static void launch_fast_t(const StepArgs &b, dim3 grid, dim3 block,
+                          hipStream_t s) {
+  if (b.deep)
+    hipLaunchKernelGGL((k_fast<TX, TY, true, FS, CS>), grid, block, 0, s, b);
+  else
+    hipLaunchKernelGGL((k_fast<TX, TY, false, FS, CS>), grid, block, 0, s, b);
+}
+
+template <int TX, int TY>
+static void launch_fast_tile(const StepArgs &b, dim3 grid, dim3 block,
+                             hipStream_t s) {
+#define DCX_FORM(FS, CS)                                                       \
+  if (b.fstore == FS && b.cstore == CS)                                        \
+    return launch_fast_t<TX, TY, FS, CS>(b, grid, block, s);
+  DCX_FORM(DCX_STORE_F32, DCX_COEF_F32)
+  DCX_FORM(DCX_STORE_F32, DCX_COEF_F16)
+  DCX_FORM(DCX_STORE_F32, DCX_COEF_U16)
+  DCX_FORM(DCX_STORE_F16, DCX_COEF_F32)
+  DCX_FORM(DCX_STORE_F16, DCX_COEF_F16)
+  DCX_FORM(DCX_STORE_F16, DCX_COEF_U16)
+  DCX_FORM(DCX_STORE_BF16, DCX_COEF_F16)
+  DCX_FORM(DCX_STORE_BF16, DCX_COEF_U16)
+#undef DCX_FORM
+  DCX_FAIL("unsupported storage combination %d/%d", b.fstore, b.cstore);
+}
+
 void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
   const int64_t nx = a.box.x1 - a.box.x0, ny = a.box.y1 - a.box.y0,
                 nz = a.box.z1 - a.box.z0;
@@ -466,20 +584,11 @@ void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
   if (nblocks >= (int64_t(1) << 31)) DCX_FAIL("too many tiles");
   dim3 grid((unsigned)nblocks);
   dim3 block(tile_x * tile_y);
-#define DCX_LAUNCH(TX, TY)                                                     \
-  if (tile_x == TX && tile_y == TY) {                                          \
-    if (a.deep)                                                                \
-      hipLaunchKernelGGL((k_fast<TX, TY, true>), grid, block, 0, s, b);        \
-    else                                                                       \
-      hipLaunchKernelGGL((k_fast<TX, TY, false>), grid, block, 0, s, b);       \
-    DCX_CHECK(hipGetLastError());                                              \
-    return;                                                                    \
-  }
-  DCX_LAUNCH(32, 16)
-  DCX_LAUNCH(32, 32)
-  DCX_LAUNCH(64, 16)
-#undef DCX_LAUNCH
-  DCX_FAIL("unsupported tile %dx%d", tile_x, tile_y);
+  if (tile_x == 32 && tile_y == 16) launch_fast_tile<32, 16>(b, grid, block, s);
+  else if (tile_x == 32 && tile_y == 32) launch_fast_tile<32, 32>(b, grid, block, s);
+  else if (tile_x == 64 && tile_y == 16) launch_fast_tile<64, 16>(b, grid, block, s);
+  else DCX_FAIL("unsupported tile %dx%d", tile_x, tile_y);
+  DCX_CHECK(hipGetLastError());
 }
 
 // -------------------------------------------------------------- misc ----

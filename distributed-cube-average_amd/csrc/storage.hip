@@ -1,0 +1,327 @@
+// Compact storage forms of one rank's arrays (dcx_set_storage).
+//
+// Fields: the leapfrog pair (p_{n-1}, p_n) of each of p and q is re-expressed
+// as P = p_n and D = p_n - p_{n-1} (the time difference) and stored as half or
+// bfloat16 in units of a power-of-two field scale (D with an extra power of
+// two).  Rounding P each step then perturbs only the displacement; in the
+// leapfrog form the same rounding also perturbs the implied velocity, which
+// the scheme amplifies by ~1/(omega dt) in the signal band.
+// Coefficients: the ten arrays as half (power-of-two scaled per array) or as
+// 16-bit codes (affine for the six angle terms, affine in log2 for the four
+// v^2 terms).  Encoding and decoding run on the host at conversion time; the
+// kernels decode in registers.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dc_internal.hpp"
+
+namespace dcx {
+namespace {
+
+__device__ __forceinline__ uint16_t enc_dev(int bf, float v) {
+  if (bf) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+  }
+  return __builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+__device__ __forceinline__ float dec_dev(int bf, uint16_t v) {
+  if (bf) return __uint_as_float((uint32_t)v << 16);
+  return (float)__builtin_bit_cast(_Float16, v);
+}
+
+// All conversion kernels walk whole allocations (padding included) with a
+// grid-stride loop; the padding stays consistent (zeros encode to zeros).
+constexpr int kConvBlocks = 2048;
+
+__global__ void k_encode_fields(const float *pc, const float *pp, uint16_t *P0,
+                                uint16_t *P1, uint16_t *D, int64_t n, float fs,
+                                float ds, int bf) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint16_t p = enc_dev(bf, pc[i] * fs);
+    P0[i] = p;
+    P1[i] = p;  // both levels share the ghost frame
+    D[i] = enc_dev(bf, (pc[i] - pp[i]) * fs * ds);
+  }
+}
+
+__global__ void k_decode_fields(const uint16_t *P, const uint16_t *D, float *pc,
+                                float *pp, int64_t n, float ifs, float ids,
+                                int bf) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float p = dec_dev(bf, P[i]);
+    const float d = dec_dev(bf, D[i]) * ids;
+    pc[i] = p * ifs;
+    pp[i] = (p - d) * ifs;
+  }
+}
+
+// Per-block (min, max, min positive, max positive) of a float array.
+__global__ void __launch_bounds__(256)
+    k_minmax(const float *a, int64_t n, float *out) {
+  float lo = INFINITY, hi = -INFINITY, plo = INFINITY, phi = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = a[i];
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+    if (v > 0.0f) {
+      plo = fminf(plo, v);
+      phi = fmaxf(phi, v);
+    }
+  }
+  __shared__ float s[4][256];
+  s[0][threadIdx.x] = lo;
+  s[1][threadIdx.x] = hi;
+  s[2][threadIdx.x] = plo;
+  s[3][threadIdx.x] = phi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s[0][threadIdx.x] = fminf(s[0][threadIdx.x], s[0][threadIdx.x + w]);
+      s[1][threadIdx.x] = fmaxf(s[1][threadIdx.x], s[1][threadIdx.x + w]);
+      s[2][threadIdx.x] = fminf(s[2][threadIdx.x], s[2][threadIdx.x + w]);
+      s[3][threadIdx.x] = fmaxf(s[3][threadIdx.x], s[3][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// mode 0: half of a*scale; 1: round((a-lo)/step); 2: 0 for a <= 0, else
+// 1 + round((log2 a - lo)/step).  Codes saturate to [0, 65535].
+__global__ void k_encode_coef(const float *a, uint16_t *out, int64_t n,
+                              int mode, double lo, double step, float scale) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = a[i];
+    uint16_t c;
+    if (mode == 0) {
+      c = __builtin_bit_cast(uint16_t, (_Float16)(v * scale));
+    } else {
+      double t;
+      if (mode == 1) {
+        t = ((double)v - lo) / step;
+      } else {
+        t = v > 0.0f ? 1.0 + (log2((double)v) - lo) / step : 0.0;
+      }
+      t = t < 0.0 ? 0.0 : (t > 65535.0 ? 65535.0 : t);
+      c = (uint16_t)__double2int_rn(t);
+    }
+    out[i] = c;
+  }
+}
+
+uint16_t *alloc16(const State *st) {
+  uint16_t *p = nullptr;
+  const size_t bytes = (size_t)st->alloc_floats * 2;
+  DCX_CHECK(hipMalloc(&p, bytes));
+  DCX_CHECK(hipMemsetAsync(p, 0, bytes, st->stream));
+  return p + kPad;
+}
+
+void free16(uint16_t *&p) {
+  if (p) DCX_CHECK(hipFree(p - kPad));
+  p = nullptr;
+}
+
+// Whole-allocation views (the pointers above are offset by kPad).
+template <class T>
+T *whole(T *p) {
+  return p - kPad;
+}
+
+void decode_into(State *st, int f, float *pc, float *pp) {
+  hipLaunchKernelGGL(k_decode_fields, dim3(kConvBlocks), dim3(256), 0,
+                     st->stream, whole(st->P16[f][st->pcur]), whole(st->D16[f]),
+                     whole(pc), whole(pp), st->alloc_floats, 1.0f / st->fscale,
+                     1.0f / st->dscale, st->fstore == DCX_STORE_BF16 ? 1 : 0);
+  DCX_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void storage_free(State *st) {
+  for (int f = 0; f < 2; ++f) {
+    free16(st->P16[f][0]);
+    free16(st->P16[f][1]);
+    free16(st->D16[f]);
+  }
+  if (st->C16) DCX_CHECK(hipFree(st->C16 - kPad));
+  st->C16 = nullptr;
+}
+
+// Host copy of field `which` (0 pp, 1 pc, 2 qp, 3 qc) of a compact struct:
+// decoded on the device into scratch arrays, then copied out.
+void storage_download(const State *st_c, int which, float *host) {
+  State *st = const_cast<State *>(st_c);
+  float *tmp = nullptr;
+  DCX_CHECK(hipMalloc(&tmp, (size_t)st->alloc_floats * 2 * sizeof(float)));
+  float *pc = tmp + kPad, *pp = tmp + st->alloc_floats + kPad;
+  decode_into(st, which / 2, pc, pp);
+  DCX_CHECK(hipMemcpy2DAsync(host, st->sizes[0] * 4, (which % 2) ? pc : pp,
+                             st->ys * 4, st->sizes[0] * 4,
+                             st->sizes[1] * st->sizes[2], hipMemcpyDeviceToHost,
+                             st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  DCX_CHECK(hipFree(tmp));
+}
+
+// Add a source amplitude (true units) at device index i of a compact struct.
+__global__ void k_add_source16(uint16_t *P0, uint16_t *P1, uint16_t *D0,
+                               uint16_t *D1, int64_t i, float amp, float ds,
+                               int bf) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint16_t *Ps[2] = {P0, P1};
+  uint16_t *Ds[2] = {D0, D1};
+  for (int f = 0; f < 2; ++f) {
+    Ps[f][i] = enc_dev(bf, dec_dev(bf, Ps[f][i]) + amp);
+    Ds[f][i] = enc_dev(bf, dec_dev(bf, Ds[f][i]) + amp * ds);
+  }
+}
+
+void storage_add_source(State *st, int64_t dev_index, float amp) {
+  hipLaunchKernelGGL(k_add_source16, dim3(1), dim3(64), 0, st->stream,
+                     st->P16[0][st->pcur], st->P16[1][st->pcur], st->D16[0],
+                     st->D16[1], dev_index, amp * st->fscale, st->dscale,
+                     st->fstore == DCX_STORE_BF16 ? 1 : 0);
+  DCX_CHECK(hipGetLastError());
+}
+
+}  // namespace dcx
+
+using namespace dcx;
+
+extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
+                               int coef_store, int field_scale_log2) {
+  State *st = state_of(data);
+  const bool fs_ok = field_store == DCX_STORE_F32 ||
+                     field_store == DCX_STORE_F16 ||
+                     field_store == DCX_STORE_BF16;
+  const bool cs_ok = coef_store == DCX_COEF_F32 || coef_store == DCX_COEF_F16 ||
+                     coef_store == DCX_COEF_U16;
+  if (!fs_ok || !cs_ok) return -1;
+  if (field_store == DCX_STORE_BF16 && coef_store == DCX_COEF_F32) return -1;
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+
+  // Bring the fields back to FP32 first if they are compact.
+  if (st->fstore != DCX_STORE_F32) {
+    for (int f = 0; f < 2; ++f)
+      decode_into(st, f, f == 0 ? st->abi.pc : st->abi.qc,
+                  f == 0 ? st->abi.pp : st->abi.qp);
+    DCX_CHECK(hipStreamSynchronize(st->stream));
+    for (int f = 0; f < 2; ++f) {
+      free16(st->P16[f][0]);
+      free16(st->P16[f][1]);
+      free16(st->D16[f]);
+    }
+    st->fstore = DCX_STORE_F32;
+  }
+  if (field_store != DCX_STORE_F32) {
+    st->fstore = field_store;
+    st->fscale = field_store == DCX_STORE_F16
+                     ? std::ldexp(1.0f, field_scale_log2)
+                     : 1.0f;
+    st->dscale = 8.0f;
+    st->pcur = 0;
+    for (int f = 0; f < 2; ++f) {
+      st->P16[f][0] = alloc16(st);
+      st->P16[f][1] = alloc16(st);
+      st->D16[f] = alloc16(st);
+      hipLaunchKernelGGL(k_encode_fields, dim3(kConvBlocks), dim3(256), 0,
+                         st->stream, whole(f == 0 ? st->abi.pc : st->abi.qc),
+                         whole(f == 0 ? st->abi.pp : st->abi.qp),
+                         whole(st->P16[f][0]), whole(st->P16[f][1]),
+                         whole(st->D16[f]), st->alloc_floats, st->fscale,
+                         st->dscale, field_store == DCX_STORE_BF16 ? 1 : 0);
+      DCX_CHECK(hipGetLastError());
+    }
+  }
+
+  if (st->C16) DCX_CHECK(hipFree(st->C16 - kPad));
+  st->C16 = nullptr;
+  st->cstore = coef_store;
+  if (coef_store != DCX_COEF_F32) {
+    DCX_CHECK(hipMalloc(&st->C16, (size_t)st->alloc_floats * 2 * NCOEF));
+    st->C16 += kPad;
+    const dc_precomp_vars &pv = st->abi.precomp_vars;
+    const float *cs[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
+                              pv.ch1dyz, pv.ch1dxz, pv.v2px,   pv.v2pz,
+                              pv.v2sz,   pv.v2pn};
+    float *d_mm = nullptr;
+    DCX_CHECK(hipMalloc(&d_mm, 4 * kConvBlocks * sizeof(float)));
+    std::vector<float> mm(4 * kConvBlocks);
+    for (int k = 0; k < NCOEF; ++k) {
+      hipLaunchKernelGGL(k_minmax, dim3(kConvBlocks), dim3(256), 0, st->stream,
+                         whole(cs[k]), st->alloc_floats, d_mm);
+      DCX_CHECK(hipGetLastError());
+      DCX_CHECK(hipMemcpyAsync(mm.data(), d_mm, mm.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, st->stream));
+      DCX_CHECK(hipStreamSynchronize(st->stream));
+      float lo = INFINITY, hi = -INFINITY, plo = INFINITY, phi = -INFINITY;
+      for (int b = 0; b < kConvBlocks; ++b) {
+        lo = std::min(lo, mm[4 * b]);
+        hi = std::max(hi, mm[4 * b + 1]);
+        plo = std::min(plo, mm[4 * b + 2]);
+        phi = std::max(phi, mm[4 * b + 3]);
+      }
+      int mode;
+      double elo = 0.0, estep = 1.0;
+      float scale = 1.0f;
+      if (coef_store == DCX_COEF_F16) {
+        const float m = std::max(std::fabs(lo), std::fabs(hi));
+        // Largest magnitude scaled to [2^13, 2^14): far from half's overflow.
+        const int e = m > 0 ? 13 - (int)std::floor(std::log2(m)) : 0;
+        scale = std::ldexp(1.0f, e);
+        mode = 0;
+        st->c_lo[k] = 0.0f;
+        st->c_st[k] = std::ldexp(1.0f, -e);
+      } else if (k < V2PX) {
+        mode = 1;
+        elo = lo;
+        estep = hi > lo ? ((double)hi - lo) / 65535.0 : 1.0;
+        st->c_lo[k] = (float)elo;
+        st->c_st[k] = (float)estep;
+      } else {
+        if (lo < 0) DCX_FAIL("coefficient %d has negative values", k);
+        mode = 2;
+        double l0 = 0.0, l1 = 0.0;
+        if (plo <= phi) {
+          l0 = std::log2((double)plo);
+          l1 = std::log2((double)phi);
+        }
+        // Codes 1..65535 span [l0, l1] in log2; code 0 is exactly zero.
+        estep = l1 > l0 ? (l1 - l0) / 65534.0 : 1.0;
+        elo = l0;
+        st->c_lo[k] = (float)(l0 - estep);
+        st->c_st[k] = (float)estep;
+      }
+      hipLaunchKernelGGL(k_encode_coef, dim3(kConvBlocks), dim3(256), 0,
+                         st->stream, whole(cs[k]),
+                         whole(st->C16 + (size_t)k * st->alloc_floats),
+                         st->alloc_floats, mode, elo, estep, scale);
+      DCX_CHECK(hipGetLastError());
+    }
+    DCX_CHECK(hipStreamSynchronize(st->stream));
+    DCX_CHECK(hipFree(d_mm));
+  }
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  build_halo_tables(st);
+  return 0;
+}
+
+extern "C" int dcx_bytes_per_cell(const dc_device_data *data) {
+  const State *st = state_of(data);
+  const int fb = st->fstore == DCX_STORE_F32 ? 4 : 2;
+  const int cb = st->cstore == DCX_COEF_F32 ? 4 : 2;
+  // Leapfrog: read pc, qc, pp, qp, write pp, qp.  Time difference: read P and
+  // D, write P and D, per field.
+  const int fields = st->fstore == DCX_STORE_F32 ? 6 * fb : 8 * fb;
+  return fields + NCOEF * cb;
+}

@@ -27,6 +27,8 @@ COEF_NAMES = ("ch1dxx", "ch1dyy", "ch1dzz", "ch1dxy", "ch1dyz", "ch1dxz",
               "v2px", "v2pz", "v2sz", "v2pn")
 KERNEL_FAST = 0
 KERNEL_EXACT = 1
+STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+COEF_F32, COEF_F16, COEF_U16 = 0, 1, 2
 
 _FP = C.POINTER(C.c_float)
 _SZ3 = C.c_size_t * 3
@@ -93,12 +95,18 @@ SIGNATURES = {
     "dcx_set_xcd_remap": (C.c_int, [_DD, C.c_int]),
     "dcx_set_prefetch_depth": (C.c_int, [_DD, C.c_int]),
     "dcx_synchronize": (None, [_DD]),
+    "dcx_set_storage": (C.c_int, [_DD, C.c_int, C.c_int, C.c_int]),
+    "dcx_bytes_per_cell": (C.c_int, [_DD]),
     "dcx_download": (None, [_DD, C.c_int, _FP]),
     "dcx_upload": (None, [_DD, C.c_int, _FP]),
     "dcx_step": (None, [_DD, C.c_float, C.c_float, C.c_float, C.c_float,
                         C.c_longlong, C.c_float, C.c_int]),
     "dcx_run": (C.c_double, [_DD, C.c_float, C.c_float, C.c_float, C.c_float,
                              C.c_uint, C.c_longlong, _FP, C.c_int]),
+    "dcx_run_traces": (C.c_double, [_DD, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_uint, C.c_longlong, _FP,
+                                    C.c_int, C.POINTER(C.c_longlong), _FP,
+                                    C.c_int]),
     "dcx_set_timing_detail": (None, [_DD, C.c_int]),
     "dcx_last_kernel_ms": (C.c_double, [_DD]),
     "dcx_pack_boxes": (None, [_DD, _FP, C.c_int, C.POINTER(C.c_size_t),
@@ -274,6 +282,15 @@ class DeviceData:
     def set_xcd_remap(self, on: bool) -> bool:
         return bool(load().dcx_set_xcd_remap(self.ptr, int(on)))
 
+    def set_storage(self, field_store: int, coef_store: int,
+                    field_scale_log2: int = 8) -> bool:
+        """Switch to a compact storage form (see include/dc_hip.h)."""
+        return load().dcx_set_storage(self.ptr, field_store, coef_store,
+                                      field_scale_log2) == 0
+
+    def bytes_per_cell(self) -> int:
+        return load().dcx_bytes_per_cell(self.ptr)
+
     def synchronize(self):
         load().dcx_synchronize(self.ptr)
 
@@ -317,6 +334,19 @@ class DeviceData:
     def copy_field(self, dst: int, src: int) -> int:
         """Calibration copy of a whole allocation; returns bytes read."""
         return load().dcx_copy_field(self.ptr, dst, src)
+
+    def run_traces(self, n_steps: int, src_index, src_amp, receivers,
+                   graph=False):
+        """run() that also returns p at the receivers after every step,
+        shaped (n_steps, n_receivers); returns (traces, elapsed ms)."""
+        amp = np.ascontiguousarray(src_amp, np.float32)
+        rec = np.ascontiguousarray(receivers, np.int64)
+        tr = np.zeros((n_steps, rec.size), np.float32)
+        ms = load().dcx_run_traces(
+            self.ptr, self.dx, self.dy, self.dz, self.dt, n_steps, src_index,
+            fp(amp), rec.size, rec.ctypes.data_as(C.POINTER(C.c_longlong)),
+            fp(tr), int(graph))
+        return tr, ms
 
     def set_timing_detail(self, on: bool):
         load().dcx_set_timing_detail(self.ptr, int(on))

@@ -154,6 +154,30 @@ def layered_octant(seed=0, local=512, absorption=32, d=20.0, steps=1000,
                     dict(seed=seed, local=local, absorption=absorption))
 
 
+def crop(w: Workload, origin, size, receiver_z=None):
+    """A size^3 window of w's arrays starting at origin (x, y, z), keeping dt
+    and the source (which must fall inside).  Receivers: every compute cell
+    of plane receiver_z (window coordinates) when given."""
+    x0, y0, z0 = origin
+    sl = (slice(z0, z0 + size), slice(y0, y0 + size), slice(x0, x0 + size))
+    coef = [np.ascontiguousarray(c[sl]) for c in w.coef]
+    sx, sy, _ = w.sizes
+    s = w.src_index
+    sxg, syg, szg = s % sx, (s // sx) % sy, s // (sx * sy)
+    lx, ly, lz = sxg - x0, syg - y0, szg - z0
+    if not all(STENCIL <= v < size - STENCIL for v in (lx, ly, lz)):
+        raise ValueError("source outside the crop's compute region")
+    sizes = (size, size, size)
+    rec = np.zeros(0, np.int64)
+    if receiver_z is not None:
+        xs = np.arange(STENCIL, size - STENCIL)
+        gx, gy = np.meshgrid(xs, xs)
+        rec = (gx + gy * size + receiver_z * size * size).ravel().astype(np.int64)
+    return Workload(w.name + f"_crop{size}", coef, w.dx, w.dy, w.dz, w.dt,
+                    w.steps, _index(sizes, lx, ly, lz), w.fcut, rec,
+                    dict(w.meta, crop_origin=origin))
+
+
 def random_small(shape, seed=0, d=20.0, steps=20, peak_hz=20.0, cfl_frac=0.4):
     """Unfiltered random TTI model on a small grid (parity tests)."""
     rng = np.random.default_rng(seed)

@@ -58,6 +58,37 @@ void dcx_synchronize(dc_device_data *data);
 void dcx_download(dc_device_data *data, int which, float *host);
 void dcx_upload(dc_device_data *data, int which, const float *host);
 
+/* ----------------------------------------------------- storage forms ---- */
+
+/* Field storage:
+ *  DCX_STORE_F32   leapfrog on the FP32 pp/pc/qp/qc arrays (the reference's
+ *                  form; the only one dc_propagate accepts).
+ *  DCX_STORE_F16   time-difference form: P = p_n and D = p_n - p_{n-1} per
+ *                  field, both stored as IEEE half in units of a power-of-two
+ *                  field scale (D with its own power-of-two factor); the
+ *                  update D += dt^2 rhs, P += D runs in FP32 registers.
+ *  DCX_STORE_BF16  the same with bfloat16 storage.
+ * Coefficient storage:
+ *  DCX_COEF_F32    the ten FP32 arrays.
+ *  DCX_COEF_F16    the ten arrays as half, each scaled by a power of two.
+ *  DCX_COEF_U16    16-bit codes per array: affine over [min,max] for the six
+ *                  angle terms, affine in log2 for the four v^2 terms (code
+ *                  0 = exactly zero); ~14x finer than half on v^2. */
+enum { DCX_STORE_F32 = 0, DCX_STORE_F16 = 1, DCX_STORE_BF16 = 2 };
+enum { DCX_COEF_F32 = 0, DCX_COEF_F16 = 1, DCX_COEF_U16 = 2 };
+
+/* Convert the struct to the given storage (encoding the current fields and
+ * coefficients; field_scale_log2 sets the field scale 2^k for half storage).
+ * dcx_step / dcx_run then run the matching kernel; dcx_download,
+ * dc_device_data_get_results and going back to DCX_STORE_F32 decode.  The
+ * fast kernel only.  Returns 0, or -1 for an unknown combination. */
+int dcx_set_storage(dc_device_data *data, int field_store, int coef_store,
+                    int field_scale_log2);
+
+/* Bytes per cell-update the fast kernel moves (algorithmic, no apron) for
+ * the struct's current storage. */
+int dcx_bytes_per_cell(const dc_device_data *data);
+
 /* ------------------------------------------------------ fused steps ---- */
 
 /* One whole time step of one rank with no neighbours, in one launch:
@@ -78,6 +109,15 @@ void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
 double dcx_run(dc_device_data *data, float dx, float dy, float dz, float dt,
                unsigned int n_steps, long long src_index,
                const float *src_amp, int use_graph);
+
+/* dcx_run that also samples p at n_rec receiver cells (linear indices
+ * x + y*sx + z*sx*sy of the caller's layout) after every step:
+ * traces[step * n_rec + r] (n_steps * n_rec floats, host). */
+double dcx_run_traces(dc_device_data *data, float dx, float dy, float dz,
+                      float dt, unsigned int n_steps, long long src_index,
+                      const float *src_amp, int n_rec,
+                      const long long *receivers, float *traces,
+                      int use_graph);
 
 /* Average duration (ms) of the stencil launches of the last dcx_run, from
  * HIP events recorded around each launch when timing_detail was enabled. */

@@ -331,6 +331,20 @@ void orc_run_f32(const size_t sizes[3], float *f[4],
   }
 }
 
+/* orc_run_f32 that also records pc at n_rec receiver cells (linear
+ * indices rec[]) after every step into traces[step * n_rec + r]. */
+void orc_run_f32_traces(const size_t sizes[3], float *f[4],
+                        const float *const coef[10], float dx, float dy,
+                        float dz, float dt, unsigned int n_steps,
+                        long long src_index, const float *src_amp, int n_rec,
+                        const long long *rec, float *traces) {
+  for (unsigned int it = 0; it < n_steps; it++) {
+    orc_run_f32(sizes, f, coef, dx, dy, dz, dt, 1, src_index,
+                src_amp ? src_amp + it : NULL);
+    for (int r = 0; r < n_rec; r++) traces[(size_t)it * n_rec + r] = f[1][rec[r]];
+  }
+}
+
 /* ---------------------------------------------------------------- FP64 ---- */
 
 static inline double o_d2d(const double *f, ptrdiff_t i, ptrdiff_t s,
@@ -423,5 +437,18 @@ void orc_run_f64(const size_t sizes[3], double *f[4],
     t = f[2];
     f[2] = f[3];
     f[3] = t;
+  }
+}
+
+/* orc_run_f64 recording pc at the receivers after every step. */
+void orc_run_f64_traces(const size_t sizes[3], double *f[4],
+                        const float *const coef[10], double dx, double dy,
+                        double dz, double dt, unsigned int n_steps,
+                        long long src_index, const float *src_amp, int n_rec,
+                        const long long *rec, double *traces) {
+  for (unsigned int it = 0; it < n_steps; it++) {
+    orc_run_f64(sizes, f, coef, dx, dy, dz, dt, 1, src_index,
+                src_amp ? src_amp + it : NULL);
+    for (int r = 0; r < n_rec; r++) traces[(size_t)it * n_rec + r] = f[1][rec[r]];
   }
 }

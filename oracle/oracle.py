@@ -55,6 +55,9 @@ def lib() -> C.CDLL:
         L.orc_run_f64.argtypes = [_SZ3, _DP * 4, _FP * 10, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_uint,
                                   C.c_longlong, _FP]
+        _LLP = C.POINTER(C.c_longlong)
+        L.orc_run_f32_traces.argtypes = L.orc_run_f32.argtypes + [C.c_int, _LLP, _FP]
+        L.orc_run_f64_traces.argtypes = L.orc_run_f64.argtypes + [C.c_int, _LLP, _DP]
         L.orc_source_f32.argtypes = [C.c_float, C.c_int, C.c_double]
         L.orc_source_f32.restype = C.c_float
         L.orc_default_model.argtypes = [C.c_size_t] + [_FP] * 6
@@ -149,6 +152,25 @@ def run_f64(fields, coef, dx, dy, dz, dt, n_steps, src_index=-1, src_amp=None):
                       dt, n_steps, src_index, None if amp is None else fp(amp))
     addr = {a.ctypes.data: a for a in f}
     return [addr[C.cast(ptrs[k], C.c_void_p).value] for k in range(4)]
+
+
+def run_traces(fields, coef, dx, dy, dz, dt, n_steps, src_index, src_amp,
+               receivers, double=False):
+    """run_f32/run_f64 that also returns pc at the receivers after each
+    step, shaped (n_steps, n_receivers)."""
+    dt_ = np.float64 if double else np.float32
+    P = _DP if double else _FP
+    f = [np.ascontiguousarray(a, dt_).copy() for a in fields]
+    ptrs = (P * 4)(*[a.ctypes.data_as(P) for a in f])
+    rec = np.ascontiguousarray(receivers, np.int64)
+    tr = np.zeros((n_steps, rec.size), dt_)
+    amp = np.ascontiguousarray(src_amp, np.float32)
+    fn = lib().orc_run_f64_traces if double else lib().orc_run_f32_traces
+    fn(_sz3(sizes_of(f[0])), ptrs, _coef_arr(coef), dx, dy, dz, dt, n_steps,
+       src_index, fp(amp), rec.size, rec.ctypes.data_as(C.POINTER(C.c_longlong)),
+       tr.ctypes.data_as(P))
+    addr = {a.ctypes.data: a for a in f}
+    return [addr[C.cast(ptrs[k], C.c_void_p).value] for k in range(4)], tr
 
 
 def default_model(n: int):

@@ -1,0 +1,69 @@
+"""Compact storage forms (dcx_set_storage) against the FP64 oracle.
+
+FP32 fields with 16-bit coefficient codes must stay within 1e-4 of FP64 on the
+40-step fixture; half/bfloat16 field storage is a precision trade-off whose
+error is printed and bounded loosely here (the study at 1000-4000 steps is in
+bench.py --precision-study).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def run40(H, O):
+    g = golden("golden_run.npz")
+    coef = H.precompute(g["vpz"], g["vsv"], g["epsilon"], g["delta"], g["phi"],
+                        g["theta"])
+    dt, steps, src, d = float(g["dt"]), int(g["steps"]), int(g["src_index"]), float(g["dx"])
+    amp = H.ricker_table(dt, steps)
+    f64 = O.run_f64([np.zeros(g["vpz"].shape)] * 4, coef, d, d, d, dt, steps, src, amp)
+    return coef, dt, steps, src, d, amp, f64
+
+
+@pytest.mark.parametrize("form,bpc,tol", [
+    ((0, 0), 64, 1e-5), ((0, 1), 44, 1e-3), ((0, 2), 44, 1e-4),
+    ((1, 1), 36, 2e-2), ((1, 2), 36, 2e-2), ((2, 2), 36, 1e-1)])
+def test_storage_form_accuracy(H, run40, form, bpc, tol):
+    coef, dt, steps, src, d, amp, f64 = run40
+    dd = H.DeviceData(coef, None, d, d, d, dt)
+    assert dd.set_storage(*form)
+    assert dd.bytes_per_cell() == bpc
+    dd.run(steps, src, amp, graph=True)
+    pc, qc = dd.get_results()
+    dd.close()
+    e = rel_l2(pc, f64[1]), rel_l2(qc, f64[3])
+    say(f"storage {form} ({bpc} B/cell): relL2 vs FP64 p={e[0]:.3e} q={e[1]:.3e}")
+    assert max(e) < tol
+
+
+def test_storage_round_trip(H, run40):
+    """F32 -> half fields -> F32 reproduces pc exactly up to half rounding and
+    keeps stepping consistently."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    dd = H.DeviceData(coef, None, d, d, d, dt)
+    dd.run(10, src, amp[:10])
+    before = dd.fields()
+    assert dd.set_storage(1, 2, 8)
+    mid = dd.fields()
+    for a, b in zip(before, mid):
+        assert np.allclose(a, b, rtol=2e-3, atol=2e-3 * np.abs(a).max())
+    assert dd.set_storage(0, 0)
+    after = dd.fields()
+    for a, b in zip(mid, after):
+        np.testing.assert_array_equal(a, b)
+    dd.close()
