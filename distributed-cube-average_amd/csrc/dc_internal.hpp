@@ -58,6 +58,9 @@ struct StepArgs {
   // cell at plane src_z, in-plane offset src_xy (x + y*ys).
   int64_t src_z, src_xy;
   float src_amp;
+  // Batched launches read the source amplitude of step it+1 from this
+  // per-shot device table (field units) instead of src_amp.
+  const float *src_tab;
   // Storage-generic views for k_fast (set by make_step_args).  Field arrays
   // hold values in field units (true value x fscale).  Leapfrog storage:
   // sf = pc/qc (read), of = pp/qp (read, then overwritten with the new
@@ -78,6 +81,16 @@ struct StepArgs {
 
 void launch_exact(const StepArgs &a, hipStream_t s);
 void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s);
+// Number of tiles one fast launch over a's box has (tile_x x tile_y x zchunk).
+int64_t fast_tiles(const StepArgs &a, int tile_x, int tile_y);
+// One launch stepping many volumes: shots[k] (device array) with their tile
+// counts' prefix sums prefix[0..nshots] (device), `grid` persistent
+// workgroups looping over all tiles; it/n_steps select each shot's source
+// amplitude from its src_tab.  All shots share the storage form of shots[0].
+void launch_fast_batch(const StepArgs *shots, const int64_t *prefix,
+                       int nshots, int64_t total, int fstore, int cstore,
+                       int deep, int tile_x, int tile_y, int grid, int it,
+                       int n_steps, hipStream_t s);
 bool fast_tiling_supported(int tile_x, int tile_y);
 void launch_add_source(float *pc, float *qc, int64_t index, float amp,
                        hipStream_t s);

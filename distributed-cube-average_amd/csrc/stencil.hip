@@ -257,7 +257,8 @@ struct Staged {
 // wave keeps in flight (more VGPRs; same occupancy at 512 threads).
 // FS/CS select the field and coefficient storage.
 template <int TX, int TY, bool DEEP, int FS, int CS>
-__global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
+__device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
+                                          const float src_amp) {
   using F = FieldIO<FS>;
   using Cf = CoefIO<CS>;
   constexpr bool kTimeDiff = FS != DCX_STORE_F32;
@@ -279,12 +280,7 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   const int tx = tid % TX, ty = tid / TX;
   const int64_t zs = a.zs;
   const int ys = (int)a.ys;
-  // Workgroups are dealt to the 8 XCDs round-robin by linear id; remapping
-  // gives each XCD (and its L2) a contiguous run of tiles, x fastest, so the
-  // apron rows and columns a tile shares with its neighbours stay in one L2.
-  int b = blockIdx.x;
-  const int G = gridDim.x;
-  if (a.xcd_remap && G % 8 == 0) b = (b % 8) * (G / 8) + b / 8;
+  // Tile b of the volume: x fastest, then y, then z chunk.
   const int bx = b % a.gx, by = (b / a.gx) % a.gy, bz = b / (a.gx * a.gy);
   const int tx0 = (int)a.box.x0 + bx * TX;
   const int ty0 = (int)a.box.y0 + by * TY;
@@ -494,10 +490,10 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
           float dnp = opp + rhsp * dt2d, dnq = oqp + rhsq * dt2d;
           float pn = rp[J] + dnp * inv_d, qn = rq[J] + dnq * inv_d;
           if (src) {
-            pn += a.src_amp;
-            qn += a.src_amp;
-            dnp += a.src_amp * a.dscale;
-            dnq += a.src_amp * a.dscale;
+            pn += src_amp;
+            qn += src_amp;
+            dnp += src_amp * a.dscale;
+            dnq += src_amp * a.dscale;
           }
           if (act) {
             F::st(Gpn, bo, pn);
@@ -509,8 +505,8 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
           float pn = 2.0f * rp[J] - opp + rhsp * dt2;
           float qn = 2.0f * rq[J] - oqp + rhsq * dt2;
           if (src) {
-            pn += a.src_amp;
-            qn += a.src_amp;
+            pn += src_amp;
+            qn += src_amp;
           }
           if (act) {
             F::st(Gpp, bo, pn);
@@ -532,6 +528,43 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
 
       cur = nxt;
     }
+  }
+}
+
+template <int TX, int TY, bool DEEP, int FS, int CS>
+__global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
+  // Workgroups are dealt to the 8 XCDs round-robin by linear id; remapping
+  // gives each XCD (and its L2) a contiguous run of tiles, x fastest, so the
+  // apron rows and columns a tile shares with its neighbours stay in one L2.
+  int b = blockIdx.x;
+  const int G = gridDim.x;
+  if (a.xcd_remap && G % 8 == 0) b = (b % 8) * (G / 8) + b / 8;
+  fast_tile<TX, TY, DEEP, FS, CS>(a, b, a.src_amp);
+}
+
+// Persistent launch over the tiles of many volumes (shots): workgroup g takes
+// work items g, g + G, g + 2G, ... of the concatenated tile lists; every
+// workgroup runs the same number of items' worth of barriers on its own.
+template <int TX, int TY, bool DEEP, int FS, int CS>
+__global__ void __launch_bounds__(TX *TY)
+    k_fast_batch(const StepArgs *__restrict__ shots,
+                 const int64_t *__restrict__ prefix, int nshots,
+                 int64_t total, int it, int n_steps) {
+  for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    int lo = 0, hi = nshots - 1;  // last shot with prefix <= w
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= w)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int k = __builtin_amdgcn_readfirstlane(lo);
+    const StepArgs &a = shots[k];
+    const float amp = (a.src_z >= 0 && a.src_tab != nullptr && it + 1 < n_steps)
+                          ? a.src_tab[it + 1]
+                          : 0.0f;
+    fast_tile<TX, TY, DEEP, FS, CS>(a, (int)(w - prefix[k]), amp);
   }
 }
 
@@ -565,6 +598,51 @@ static void launch_fast_tile(const StepArgs &b, dim3 grid, dim3 block,
   DCX_FORM(DCX_STORE_BF16, DCX_COEF_U16)
 #undef DCX_FORM
   DCX_FAIL("unsupported storage combination %d/%d", b.fstore, b.cstore);
+}
+
+int64_t fast_tiles(const StepArgs &a, int tile_x, int tile_y) {
+  const int64_t nx = a.box.x1 - a.box.x0, ny = a.box.y1 - a.box.y0,
+                nz = a.box.z1 - a.box.z0;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return ((nx + tile_x - 1) / tile_x) * ((ny + tile_y - 1) / tile_y) *
+         ((nz + a.zchunk - 1) / a.zchunk);
+}
+
+template <int TX, int TY, int FS, int CS>
+static void launch_batch_t(const StepArgs *shots, const int64_t *prefix,
+                           int nshots, int64_t total, int deep, dim3 grid,
+                           int it, int n_steps, hipStream_t s) {
+  if (deep)
+    hipLaunchKernelGGL((k_fast_batch<TX, TY, true, FS, CS>), grid,
+                       dim3(TX * TY), 0, s, shots, prefix, nshots, total, it,
+                       n_steps);
+  else
+    hipLaunchKernelGGL((k_fast_batch<TX, TY, false, FS, CS>), grid,
+                       dim3(TX * TY), 0, s, shots, prefix, nshots, total, it,
+                       n_steps);
+}
+
+void launch_fast_batch(const StepArgs *shots, const int64_t *prefix,
+                       int nshots, int64_t total, int fstore, int cstore,
+                       int deep, int tile_x, int tile_y, int grid, int it,
+                       int n_steps, hipStream_t s) {
+  if (total <= 0 || nshots <= 0) return;
+  if (tile_x != 32 || tile_y != 16)
+    DCX_FAIL("batched launches use 32x16 tiles (asked %dx%d)", tile_x, tile_y);
+  const dim3 g((unsigned)(grid > 0 ? grid : 1));
+#define DCX_BFORM(FS, CS)                                                      \
+  if (fstore == FS && cstore == CS) {                                          \
+    launch_batch_t<32, 16, FS, CS>(shots, prefix, nshots, total, deep, g, it,  \
+                                   n_steps, s);                                \
+    DCX_CHECK(hipGetLastError());                                              \
+    return;                                                                    \
+  }
+  DCX_BFORM(DCX_STORE_F32, DCX_COEF_F32)
+  DCX_BFORM(DCX_STORE_F32, DCX_COEF_U16)
+  DCX_BFORM(DCX_STORE_F16, DCX_COEF_F16)
+  DCX_BFORM(DCX_STORE_F16, DCX_COEF_U16)
+#undef DCX_BFORM
+  DCX_FAIL("unsupported batched storage combination %d/%d", fstore, cstore);
 }
 
 void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {

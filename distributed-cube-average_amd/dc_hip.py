@@ -116,6 +116,14 @@ SIGNATURES = {
     "dcx_scratch": (_FP, [_DD, C.c_size_t]),
     "dcx_set_halo_loopback": (None, [_DD, C.c_int] + [C.POINTER(C.c_size_t)] * 4),
     "dcx_copy_field": (C.c_size_t, [_DD, C.c_int, C.c_int]),
+    "dcx_batch_create": (C.c_void_p, [C.POINTER(_DD), C.c_int]),
+    "dcx_batch_free": (None, [C.c_void_p]),
+    "dcx_batch_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "dcx_batch_set_autotune": (C.c_int, [C.c_void_p, C.c_int]),
+    "dcx_batch_get_tiling": (None, [C.c_void_p, _I3]),
+    "dcx_batch_last_step_ms": (C.c_double, [C.c_void_p]),
+    "dcx_batch_run": (C.c_double, [C.c_void_p, C.c_uint, _FP,
+                                   C.POINTER(C.c_longlong), _FP, C.c_int]),
     "dcx_program_run": (C.c_int, [C.POINTER(ProgramArgs), _FP, _FP,
                                   C.POINTER(C.c_double)]),
     "dcx_write_dc_file": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t,
@@ -354,9 +362,74 @@ class DeviceData:
     def last_kernel_ms(self) -> float:
         return load().dcx_last_kernel_ms(self.ptr)
 
+    def release_host(self):
+        """Drop the host copies of the initial arrays (the device holds
+        everything after dc_device_data_init)."""
+        self._coef = self._fields = None
+        self._proc.precomp_vars = PrecompVars()
+        self._proc.pp = self._proc.pc = self._proc.qp = self._proc.qc = None
+
     def close(self):
         if getattr(self, "ptr", None):
             load().dc_device_data_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Shots (DeviceData objects, one storage form) advanced together by one
+    persistent launch per step (dcx_batch_run).  The shots stay usable; they
+    move back to streams of their own when the batch is closed."""
+
+    def __init__(self, shots: Sequence[DeviceData]):
+        self.shots = list(shots)
+        arr = (_DD * len(self.shots))(*[s.ptr for s in self.shots])
+        self.ptr = load().dcx_batch_create(arr, len(self.shots))
+
+    def set_tiling(self, z_chunk: int, prefetch_planes: int = 1,
+                   blocks_per_cu: int = 1) -> bool:
+        return load().dcx_batch_set_tiling(self.ptr, z_chunk, prefetch_planes,
+                                           blocks_per_cu) == 0
+
+    def set_autotune(self, on: bool = True):
+        load().dcx_batch_set_autotune(self.ptr, int(on))
+
+    def tiling(self):
+        """(z chunk, prefetch planes, workgroups per CU; 0 = one per tile)."""
+        out = _I3()
+        load().dcx_batch_get_tiling(self.ptr, out)
+        return tuple(out)
+
+    def last_step_ms(self) -> float:
+        return load().dcx_batch_last_step_ms(self.ptr)
+
+    def run(self, n_steps: int, src_index=None, src_amp=None, graph=False):
+        """src_index: per-shot source cells (-1 for none); src_amp: per-shot
+        amplitude tables shaped (n_shots, >= n_steps).  Returns device ms."""
+        n = len(self.shots)
+        d4 = np.array([[s.dx, s.dy, s.dz, s.dt] for s in self.shots],
+                      np.float32).reshape(-1)
+        idx = amp = None
+        if src_index is not None:
+            idx = np.ascontiguousarray(src_index, np.int64)
+            if idx.size != n:
+                raise ValueError("one source index per shot")
+            amp = np.ascontiguousarray(np.asarray(src_amp, np.float32)[:, :n_steps])
+            if amp.shape != (n, n_steps):
+                raise ValueError("source tables shorter than n_steps")
+        return load().dcx_batch_run(
+            self.ptr, n_steps, fp(d4),
+            None if idx is None else idx.ctypes.data_as(C.POINTER(C.c_longlong)),
+            None if amp is None else fp(amp), int(graph))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            load().dcx_batch_free(self.ptr)
             self.ptr = None
 
     def __del__(self):

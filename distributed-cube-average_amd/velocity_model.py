@@ -190,3 +190,88 @@ def random_small(shape, seed=0, d=20.0, steps=20, peak_hz=20.0, cfl_frac=0.4):
     return Workload("random_small", coef, d, d, d, float(np.float32(dt)), steps,
                     _index(sizes, sizes[0] // 2, sizes[1] // 2, sizes[2] // 2),
                     3 * peak_hz)
+
+
+class Survey:
+    """Independent shots over one shared model (the batched workload).  The
+    shared model stays in host memory; shot(k) cuts shot k's aperture and
+    builds its coefficients on demand, so the shots' arrays need not all be
+    held on the host at once."""
+
+    def __init__(self, model, specs, dt, d, steps, absorption, meta):
+        self.model = model      # vpz, eps, dlt, theta, phi (shared)
+        self.specs = specs      # per shot: sizes, origin, source xy, peak Hz
+        self.dt, self.d, self.steps = dt, d, steps
+        self.absorption = absorption
+        self.meta = meta
+
+    def __len__(self):
+        return len(self.specs)
+
+    def compute_cells(self) -> int:
+        return sum((sx - 2 * STENCIL) * (sy - 2 * STENCIL) * (sz - 2 * STENCIL)
+                   for (sx, sy, sz), *_ in self.specs)
+
+    def shot(self, k) -> "Workload":
+        (sx, sy, sz), (x0, y0), (px, py), f = self.specs[k]
+        vpz, eps, dlt, theta, phi = self.model
+        sl = (slice(0, sz), slice(y0, y0 + sy), slice(x0, x0 + sx))
+        v = np.ascontiguousarray(vpz[sl])
+        vs = (0.5 * v).astype(np.float32)
+        a = self.absorption
+        problem = tuple(n - 2 * STENCIL - 2 * a for n in (sx, sy, sz))
+        dc_hip.random_boundary((sx, sy, sz), (sx, sy, sz), (0, 0, 0), problem,
+                               STENCIL, a, v, vs, 0)
+        coef = dc_hip.precompute(v, vs, np.ascontiguousarray(eps[sl]),
+                                 np.ascontiguousarray(dlt[sl]),
+                                 np.ascontiguousarray(phi[sl]),
+                                 np.ascontiguousarray(theta[sl]))
+        top = STENCIL + a
+        return Workload(f"shot{k}", coef, self.d, self.d, self.d, self.dt,
+                        self.steps, _index((sx, sy, sz), px, py, top), 3 * f,
+                        meta=dict(origin=(x0, y0, 0), peak_hz=f))
+
+
+def layered_survey(seed=0, n_shots=64, lo=(128, 128, 128), hi=(320, 320, 256),
+                   lateral=640, absorption=16, d=20.0, steps=800,
+                   hz=(10.0, 25.0), cfl_frac=0.4, layers=10, noise=0.05):
+    """n_shots shots on one shared layered TTI model, each over its own
+    aperture.
+
+    Shared model: lateral x lateral x (hi_z + 2*STENCIL) cells, `layers`
+    horizontal layers 1500->4500 m/s over the depth with `noise` relative
+    seeded noise, anisotropy as smooth_random.  Shot k: compute extents
+    (nx, ny, nz) drawn uniformly from [lo, hi] per axis (seeded), an aperture
+    at a seeded lateral position, starting at the surface; the reference's
+    random-velocity absorbing shell (`absorption` cells) on all six faces of
+    the aperture; a Ricker of peak frequency U[hz] at a random surface
+    position (first plane below the shell) inside it.  One dt for all shots
+    (cfl_frac of the whole model's CFL limit).
+    """
+    rng = np.random.default_rng(seed)
+    depth = hi[2] + 2 * STENCIL
+    shape = (depth, lateral, lateral)
+    z = np.arange(depth, dtype=np.float64)
+    layer = np.clip(((z - STENCIL) * layers) // hi[2], 0, layers - 1)
+    v_layer = 1500.0 + (4500.0 - 1500.0) * layer / (layers - 1)
+    vpz = np.empty(shape, np.float32)
+    vpz[:] = v_layer[:, None, None].astype(np.float32)
+    vpz *= (1.0 + noise * rng.standard_normal(shape, dtype=np.float32))
+    eps, dlt, theta, phi = _anisotropy(rng, shape)
+    vmax2 = float(np.max(vpz)) ** 2 * (1.0 + 2.0 * float(np.max(eps)))
+    dt = float(np.float32(cfl_frac * cfl_dt(np.array([vmax2, 3000.0 ** 2]),
+                                            d, d, d)))
+    specs = []
+    top = STENCIL + absorption
+    for _ in range(n_shots):
+        nx, ny, nz = (int(rng.integers(lo[a], hi[a] + 1)) for a in range(3))
+        sx, sy, sz = nx + 2 * STENCIL, ny + 2 * STENCIL, nz + 2 * STENCIL
+        x0 = int(rng.integers(0, lateral - sx + 1))
+        y0 = int(rng.integers(0, lateral - sy + 1))
+        px = int(rng.integers(top, sx - top))
+        py = int(rng.integers(top, sy - top))
+        specs.append(((sx, sy, sz), (x0, y0), (px, py),
+                      float(rng.uniform(*hz))))
+    return Survey((vpz, eps, dlt, theta, phi), specs, dt, d, steps, absorption,
+                  dict(seed=seed, n_shots=n_shots, lo=lo, hi=hi,
+                       lateral=lateral, absorption=absorption))

@@ -47,7 +47,8 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
 int dcx_set_prefetch_depth(dc_device_data *data, int planes);
 
 /* Fast kernel: deal tiles to the GPU's 8 XCDs in contiguous runs so that
- * neighbouring tiles share an L2 (default on).  Returns the previous value. */
+ * neighbouring tiles share an L2 (default off: measured slower on the
+ * 512^3 block).  Returns the previous value. */
 int dcx_set_xcd_remap(dc_device_data *data, int enabled);
 
 /* Wait for all queued work of this struct. */
@@ -158,6 +159,49 @@ size_t dcx_copy_field(dc_device_data *data, int dst, int src);
 /* Device scratch buffer owned by the struct (grown on demand, freed with the
  * struct), for the packed halos. */
 float *dcx_scratch(dc_device_data *data, size_t n_floats);
+
+/* ------------------------------------------------------ batched shots -- */
+
+/* Many independent one-rank volumes ("shots": a survey's sources, each with
+ * its own aperture of the model) advanced together.  Each shot is a
+ * dc_device_data of its own (its own sizes, coefficients and fields; one
+ * storage form for all; no halo loopback); dcx_batch_create moves them all
+ * onto one stream of the batch, dcx_batch_free gives them streams of their
+ * own again.  No reference counterpart: the reference runs one shot per
+ * program (src/main.c). */
+typedef struct dcx_batch dcx_batch;
+dcx_batch *dcx_batch_create(dc_device_data **shots, int n);
+void dcx_batch_free(dcx_batch *batch);
+
+/* Z chunk of the 32x16-cell tiles, prefetch depth (1 or 2 planes) and
+ * persistent workgroups per CU of the batched launch (1..64; 0 = one
+ * workgroup per tile).  Returns 0, or -1 for an unsupported value (nothing
+ * changed). */
+int dcx_batch_set_tiling(dcx_batch *batch, int z_chunk, int prefetch_planes,
+                         int blocks_per_cu);
+
+/* enabled != 0: the next dcx_batch_run picks the tiling itself, timing
+ * candidates on its own first steps (z chunk 32..512, then prefetch depth,
+ * then workgroups per CU 1, 2, 4 or one per tile; two steps each).  Every
+ * tiling computes the same values, so the run's results do not depend on
+ * it.  The choice stays set for later runs (dcx_batch_get_tiling). */
+int dcx_batch_set_autotune(dcx_batch *batch, int enabled);
+/* out = {z chunk, prefetch planes, workgroups per CU (0 = one per tile)}. */
+void dcx_batch_get_tiling(const dcx_batch *batch, int out[3]);
+/* Device milliseconds per step of the last dcx_batch_run's steps after the
+ * autotuning trials. */
+double dcx_batch_last_step_ms(const dcx_batch *batch);
+
+/* n_steps full time steps of every shot: per step ONE persistent launch that
+ * walks the tiles of all shots.  d4[4k..4k+3] = dx, dy, dz, dt of shot k;
+ * src_index[k] = shot k's source cell (linear index of its arrays, < 0 for
+ * none; the array may be NULL), src_amp[k * n_steps + it] = its amplitude at
+ * step it.  Each shot's fields end as n_steps dcx_step calls on it alone
+ * would leave them, bit for bit.  use_graph != 0 captures the loop into one
+ * HIP graph.  Returns the elapsed milliseconds on the device. */
+double dcx_batch_run(dcx_batch *batch, unsigned int n_steps, const float *d4,
+                     const long long *src_index, const float *src_amp,
+                     int use_graph);
 
 /* ---------------------------------------------- whole program, 1 GPU ---- */
 
