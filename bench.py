@@ -192,6 +192,8 @@ def precision_study(H, VM, steps_list, log_prefix="[bench]"):
         t = time.time()
         f64, tr64 = O.run_traces(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, steps,
                                  w.src_index, amp, w.receivers, double=True)
+        log(f"{log_prefix} precision: FP64 oracle {steps} steps "
+            f"in {time.time() - t:.0f}s")
         f32, tr32 = O.run_traces(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, steps,
                                  w.src_index, amp, w.receivers)
         log(f"{log_prefix} precision: FP64/FP32 oracle {steps} steps "
@@ -217,6 +219,80 @@ def precision_study(H, VM, steps_list, log_prefix="[bench]"):
         res[str(steps)] = {k: {kk: float(f"{vv:.4g}") for kk, vv in v.items()}
                            for k, v in row.items()}
     return res
+
+
+BATCH_FORMS = [("fp32", 0, 0), ("fp16", 1, 1)]
+
+
+def batch_study(H, VM, args):
+    """The batched form: a survey of independent shots of uneven apertures
+    on one shared layered model (velocity_model.layered_survey), every shot
+    stepped args.batch_steps times with its own Ricker.
+
+    Compared, for each storage form, on device time of the same steps:
+    every shot run alone (dcx_run with a HIP graph, one shot after another)
+    with the library's default z chunk (512) and with 64-plane chunks, and
+    all shots in one persistent launch per step (dcx_batch_run, autotuned on
+    its own first steps, HIP graph).  The batched shots' fields are bitwise
+    those of the shots run alone (tests/test_gpu_batch.py)."""
+    t = time.time()
+    S = VM.layered_survey(seed=args.seed, n_shots=args.batch_shots,
+                          steps=args.batch_steps)
+    log(f"[bench] batch: survey of {len(S)} shots, "
+        f"{S.compute_cells() / 1e6:.1f} M compute cells per step")
+    ds, idx, amps = [], [], []
+    for k in range(len(S)):
+        w = S.shot(k)
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        d.release_host()
+        ds.append(d)
+        idx.append(w.src_index)
+        amps.append(w.source_table())
+        if k % 8 == 7 or k == len(S) - 1:
+            log(f"[bench] batch: {k + 1} shots on the device "
+                f"({time.time() - t:.0f}s)")
+    cells = S.compute_cells()
+    steps = args.batch_steps
+    amp = np.stack(amps)
+    out = {"shots": len(S), "steps": steps, "compute_cells_per_step": cells,
+           "apertures": "compute extents U[128,320]x U[128,320]x U[128,256], "
+                        "16-cell absorbing shell, 10-25 Hz Ricker at a random "
+                        "surface position", "seed": args.seed}
+    for name, fs, cs in BATCH_FORMS:
+        for d in ds:
+            d.set_storage(fs, cs, FIELD_SCALE_LOG2)
+        row = {}
+        for zc in (512, 64):
+            tot = 0.0
+            for d, i, a in zip(ds, idx, amps):
+                d.set_tiling(32, 16, zc)
+                tot += d.run(steps, i, a, graph=True)
+            row[f"alone_zchunk{zc}"] = round(cells * steps / tot / 1e3, 1)
+            log(f"[bench] batch {name}: shots alone (z chunk {zc}) "
+                f"{row[f'alone_zchunk{zc}']:.0f} Mcell/s")
+        b = H.Batch(ds)
+        b.set_autotune(True)
+        t0 = time.perf_counter()
+        ms = b.run(steps, idx, amp, graph=True)
+        wall = time.perf_counter() - t0
+        row["batched"] = round(cells * steps / ms / 1e3, 1)
+        row["batched_wall"] = round(cells * steps / wall / 1e6, 1)
+        row["batched_steady"] = round(cells / b.last_step_ms() / 1e3, 1)
+        row["tiling"] = dict(zip(("z_chunk", "prefetch_planes",
+                                  "workgroups_per_cu"), b.tiling()))
+        row["speedup_vs_alone_default"] = round(
+            row["batched"] / row["alone_zchunk512"], 3)
+        row["speedup_vs_alone_zchunk64"] = round(
+            row["batched"] / row["alone_zchunk64"], 3)
+        b.set_autotune(False)
+        b.close()
+        out[name] = row
+        log(f"[bench] batch {name}: batched {row['batched']:.0f} Mcell/s "
+            f"(steady {row['batched_steady']:.0f}, tiling {row['tiling']}), "
+            f"x{row['speedup_vs_alone_default']:.2f} vs shots alone")
+    for d in ds:
+        d.close()
+    return out
 
 
 def pmc_child(args):
@@ -303,6 +379,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=100)
     ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--batch-shots", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=800)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--precision-study", type=int, nargs="*", default=None,
                     help="step counts of the storage-form precision study")
     ap.add_argument("--traffic", action="store_true",
@@ -361,6 +440,9 @@ def main():
         forms = time_forms(H, w, d, args, amp, cells)
     fields = d.fields()
     d.close()
+    batch = None
+    if not args.no_batch:
+        batch = batch_study(H, VM, args)
     precision = None
     if args.precision_study:
         precision = precision_study(H, VM, args.precision_study)
@@ -398,6 +480,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": forms,
+        "batch": batch,
         "precision": precision,
     }
     print(json.dumps(line), flush=True)

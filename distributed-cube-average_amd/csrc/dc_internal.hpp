@@ -190,5 +190,9 @@ void build_halo_tables(State *st);
 // Move a struct onto a caller-owned stream (the struct's own stream is
 // destroyed); used by drivers that order several subdomains on one stream.
 void adopt_stream(State *st, hipStream_t s);
+// Host array (x fastest) <-> pitched device array through a packed device
+// buffer tmp of n floats (NULL: allocated and freed inside).
+void host_to_pitched(const State *st, float *dev, const float *host, float *tmp);
+void pitched_to_host(const State *st, float *host, const float *dev, float *tmp);
 
 }  // namespace dcx

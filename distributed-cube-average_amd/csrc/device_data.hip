@@ -137,21 +137,51 @@ static void advance(State *st) {
     st->pcur ^= 1;
 }
 
-// Host array (x fastest, rows of sx) <-> pitched device array.
-static void copy_in(const State *st, float *dev, const float *host) {
-  DCX_CHECK(hipMemcpy2DAsync(dev, st->ys * sizeof(float), host,
+// Host array (x fastest, rows of sx) <-> pitched device array.  The host side
+// moves as one contiguous copy through a packed device buffer (`tmp`, at
+// least sizes[0]*sizes[1]*sizes[2] floats, or NULL to allocate one) and the
+// pitch is applied by a device-to-device 2-D copy: a 2-D copy straight from
+// pageable host memory goes row by row and runs at a small fraction of the
+// link's rate.
+void host_to_pitched(const State *st, float *dev, const float *host, float *tmp) {
+  const size_t n = (size_t)st->n;
+  float *buf = tmp;
+  if (buf == nullptr) DCX_CHECK(hipMalloc(&buf, n * sizeof(float)));
+  DCX_CHECK(hipMemcpyAsync(buf, host, n * sizeof(float), hipMemcpyHostToDevice,
+                           st->stream));
+  DCX_CHECK(hipMemcpy2DAsync(dev, st->ys * sizeof(float), buf,
                              st->sizes[0] * sizeof(float),
                              st->sizes[0] * sizeof(float),
                              st->sizes[1] * st->sizes[2],
-                             hipMemcpyHostToDevice, st->stream));
+                             hipMemcpyDeviceToDevice, st->stream));
+  if (tmp == nullptr) {
+    DCX_CHECK(hipStreamSynchronize(st->stream));
+    DCX_CHECK(hipFree(buf));
+  }
 }
 
-static void copy_out(const State *st, float *host, const float *dev) {
-  DCX_CHECK(hipMemcpy2DAsync(host, st->sizes[0] * sizeof(float), dev,
+void pitched_to_host(const State *st, float *host, const float *dev, float *tmp) {
+  const size_t n = (size_t)st->n;
+  float *buf = tmp;
+  if (buf == nullptr) DCX_CHECK(hipMalloc(&buf, n * sizeof(float)));
+  DCX_CHECK(hipMemcpy2DAsync(buf, st->sizes[0] * sizeof(float), dev,
                              st->ys * sizeof(float),
                              st->sizes[0] * sizeof(float),
                              st->sizes[1] * st->sizes[2],
-                             hipMemcpyDeviceToHost, st->stream));
+                             hipMemcpyDeviceToDevice, st->stream));
+  DCX_CHECK(hipMemcpyAsync(host, buf, n * sizeof(float), hipMemcpyDeviceToHost,
+                           st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  if (tmp == nullptr) DCX_CHECK(hipFree(buf));
+}
+
+static void copy_in(const State *st, float *dev, const float *host,
+                    float *tmp = nullptr) {
+  host_to_pitched(st, dev, host, tmp);
+}
+
+static void copy_out(const State *st, float *host, const float *dev) {
+  pitched_to_host(st, host, dev, nullptr);
 }
 
 static float *ensure(float *&buf, size_t &cap, size_t n) {
@@ -280,11 +310,13 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   const size_t abytes = (size_t)st->alloc_floats * sizeof(float);
 
   float *host_fields[4] = {process->pp, process->pc, process->qp, process->qc};
+  float *tmp = nullptr;  // packed staging for the uploads below
+  DCX_CHECK(hipMalloc(&tmp, (size_t)st->n * sizeof(float)));
   for (int k = 0; k < 4; ++k) {
     DCX_CHECK(hipMalloc(&st->field_alloc[k], abytes));
     DCX_CHECK(hipMemsetAsync(st->field_alloc[k], 0, abytes, st->stream));
     st->field_mem[k] = st->field_alloc[k] + kPad;
-    if (host_fields[k]) copy_in(st, st->field_mem[k], host_fields[k]);
+    if (host_fields[k]) copy_in(st, st->field_mem[k], host_fields[k], tmp);
   }
   const dc_precomp_vars &pv = process->precomp_vars;
   const float *hc[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
@@ -296,9 +328,10 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   for (int k = 0; k < NCOEF; ++k) {
     if (hc[k] == nullptr) DCX_FAIL("precomp_vars array %d is NULL", k);
     dc[k] = st->coef_alloc + (size_t)k * st->alloc_floats + kPad;
-    copy_in(st, dc[k], hc[k]);
+    copy_in(st, dc[k], hc[k], tmp);
   }
   DCX_CHECK(hipStreamSynchronize(st->stream));
+  DCX_CHECK(hipFree(tmp));
 
   dc_device_data &d = st->abi;
   d.pp = st->field_mem[0];

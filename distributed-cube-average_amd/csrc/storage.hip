@@ -165,11 +165,8 @@ void storage_download(const State *st_c, int which, float *host) {
   DCX_CHECK(hipMalloc(&tmp, (size_t)st->alloc_floats * 2 * sizeof(float)));
   float *pc = tmp + kPad, *pp = tmp + st->alloc_floats + kPad;
   decode_into(st, which / 2, pc, pp);
-  DCX_CHECK(hipMemcpy2DAsync(host, st->sizes[0] * 4, (which % 2) ? pc : pp,
-                             st->ys * 4, st->sizes[0] * 4,
-                             st->sizes[1] * st->sizes[2], hipMemcpyDeviceToHost,
-                             st->stream));
-  DCX_CHECK(hipStreamSynchronize(st->stream));
+  // The packed staging reuses the other decoded array's memory.
+  pitched_to_host(st, host, (which % 2) ? pc : pp, tmp + (which % 2 ? st->alloc_floats : 0));
   DCX_CHECK(hipFree(tmp));
 }
 
