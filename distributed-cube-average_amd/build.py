@@ -30,7 +30,7 @@ HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
 HOST_FLAGS = ["-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c++17",
               "-Wall", "-I" + INC]
 
-HIP_SOURCES = ["stencil.hip", "halo.hip", "device_data.hip", "program.hip", "batch.hip",
+HIP_SOURCES = ["stencil.hip", "halo.hip", "device_data.hip", "program.hip", "batch.hip", "learned.hip",
                "storage.hip"]
 HOST_SOURCES = ["host_model.cpp"]
 

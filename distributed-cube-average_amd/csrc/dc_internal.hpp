@@ -160,6 +160,16 @@ struct State {
   uint16_t *C16;        // ten arrays of alloc_floats elements
   float c_lo[NCOEF], c_st[NCOEF];
   int halo_esize;       // element size the loopback tables were built for
+  // Learned absorbing layer (dcx_set_learned_layer): the network, the layer
+  // width, its cells (packed compute coordinates) and their corrections.
+  const void *layer_mlp;
+  int layer_w, layer_n;
+  uint32_t *layer_cells;
+  float *layer_delta;
+  // Layer cells used when the struct is the small domain of
+  // dcx_learned_sample_step.
+  int sample_w, sample_n;
+  uint32_t *sample_cells;
 };
 
 constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"
@@ -190,6 +200,16 @@ void build_halo_tables(State *st);
 // Move a struct onto a caller-owned stream (the struct's own stream is
 // destroyed); used by drivers that order several subdomains on one stream.
 void adopt_stream(State *st, hipStream_t s);
+// Learned layer hooks around a step's stencil launch (no-ops when unset):
+// corrections from the current fields, then added to the new ones.
+void layer_infer(State *st, const StepArgs &a);
+// One whole step as dcx_step runs it; after_stencil(ctx), when given, runs
+// between the stencil launch and the learned-layer corrections.
+void fused_step(State *st, float dx, float dy, float dz, float dt,
+                long long src_index, float src_next, int add_next,
+                void (*after_stencil)(void *), void *ctx);
+void layer_apply(State *st);
+void layer_free(State *st);
 // Host array (x fastest) <-> pitched device array through a packed device
 // buffer tmp of n floats (NULL: allocated and freed inside).
 void host_to_pitched(const State *st, float *dev, const float *host, float *tmp);

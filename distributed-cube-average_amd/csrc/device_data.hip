@@ -357,6 +357,7 @@ void dc_device_data_free(dc_device_data *data) {
   if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));
   if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
   std::free(st->halo_user);
+  layer_free(st);
   storage_free(st);
   DCX_CHECK(hipEventDestroy(st->ev[0]));
   DCX_CHECK(hipEventDestroy(st->ev[1]));
@@ -555,14 +556,29 @@ static void halo_loopback(State *st) {
                   st->halo_buf, 1, st->stream);
 }
 
-void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
-              long long src_index, float src_next, int add_next) {
-  State *st = state_of(data);
+}  // extern "C"
+
+namespace dcx {
+void fused_step(State *st, float dx, float dy, float dz, float dt,
+                long long src_index, float src_next, int add_next,
+                void (*after_stencil)(void *), void *ctx) {
   StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
   if (add_next) set_source(st, a, src_index, src_next);
+  layer_infer(st, a);
   launch_stencil(st, a);
+  if (after_stencil) after_stencil(ctx);
+  layer_apply(st);
   halo_loopback(st);
   advance(st);
+}
+}  // namespace dcx
+
+extern "C" {
+
+void dcx_step(dc_device_data *data, float dx, float dy, float dz, float dt,
+              long long src_index, float src_next, int add_next) {
+  fused_step(state_of(data), dx, dy, dz, dt, src_index, src_next, add_next,
+             nullptr, nullptr);
 }
 
 void dcx_set_timing_detail(dc_device_data *data, int enabled) {
@@ -634,9 +650,11 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
     for (unsigned it = 0; it < n_steps; ++it) {
       StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
       if (src && it + 1 < n_steps) set_source(st, a, src_index, src_amp[it + 1]);
+      layer_infer(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it], st->stream));
       launch_stencil(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it + 1], st->stream));
+      layer_apply(st);
       halo_loopback(st);
       advance(st);
       if (tc) gather(st, *tc, it);

@@ -124,6 +124,28 @@ SIGNATURES = {
     "dcx_batch_last_step_ms": (C.c_double, [C.c_void_p]),
     "dcx_batch_run": (C.c_double, [C.c_void_p, C.c_uint, _FP,
                                    C.POINTER(C.c_longlong), _FP, C.c_int]),
+    "dcx_mlp_nparam": (C.c_int, []),
+    "dcx_mlp_create": (C.c_void_p, [C.c_int]),
+    "dcx_mlp_free": (None, [C.c_void_p]),
+    "dcx_mlp_set_output": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+    "dcx_mlp_set_params": (None, [C.c_void_p, _FP]),
+    "dcx_mlp_get_params": (None, [C.c_void_p, _FP]),
+    "dcx_samples_create": (C.c_void_p, [C.c_int, C.c_longlong]),
+    "dcx_samples_free": (None, [C.c_void_p]),
+    "dcx_samples_count": (C.c_longlong, [C.c_void_p]),
+    "dcx_samples_clear": (None, [C.c_void_p]),
+    "dcx_samples_upload": (None, [C.c_void_p, C.c_longlong, _FP, _FP]),
+    "dcx_samples_download": (None, [C.c_void_p, C.c_longlong, C.c_longlong,
+                                    _FP, _FP, _FP, C.POINTER(C.c_uint)]),
+    "dcx_mlp_forward": (None, [C.c_void_p, C.c_void_p, _FP]),
+    "dcx_mlp_gradient": (C.c_double, [C.c_void_p, C.c_void_p, _FP]),
+    "dcx_mlp_train": (C.c_double, [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                   C.c_float, C.c_longlong, _FP]),
+    "dcx_set_learned_layer": (C.c_int, [_DD, C.c_void_p, C.c_int]),
+    "dcx_learned_sample_step": (C.c_longlong, [
+        _DD, _DD, C.c_longlong * 3, C.c_int, C.c_int, C.c_uint, C.c_void_p,
+        C.c_float, C.c_float, C.c_float, C.c_float, C.c_longlong, C.c_float,
+        C.c_int, C.c_int]),
     "dcx_program_run": (C.c_int, [C.POINTER(ProgramArgs), _FP, _FP,
                                   C.POINTER(C.c_double)]),
     "dcx_write_dc_file": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t,
@@ -437,6 +459,122 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+MLP_FREE, MLP_DAMPING = 0, 1
+
+
+class Mlp:
+    """The learned layer's network on the device (dcx_mlp_*)."""
+
+    def __init__(self, params=None, device=0, mode=MLP_FREE, sigma_max=0.0):
+        self.nparam = load().dcx_mlp_nparam()
+        self.ptr = load().dcx_mlp_create(device)
+        if params is not None:
+            self.set_params(params)
+        self.set_output(mode, sigma_max)
+
+    def set_output(self, mode: int, sigma_max: float = 0.0):
+        if load().dcx_mlp_set_output(self.ptr, mode, sigma_max) != 0:
+            raise ValueError(f"bad output form {mode}, sigma_max {sigma_max}")
+        self.mode, self.sigma_max = mode, sigma_max
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters")
+        load().dcx_mlp_set_params(self.ptr, fp(p))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        load().dcx_mlp_get_params(self.ptr, fp(out))
+        return out
+
+    def forward(self, samples: "Samples") -> np.ndarray:
+        y = np.empty((samples.count(), 2), np.float32)
+        load().dcx_mlp_forward(self.ptr, samples.ptr, fp(y))
+        return y
+
+    def gradient(self, samples: "Samples"):
+        g = np.empty(self.nparam, np.float32)
+        l = load().dcx_mlp_gradient(self.ptr, samples.ptr, fp(g))
+        return g, l
+
+    def train(self, samples: "Samples", n_iter: int, batch: int, lr: float,
+              seed: int = 0):
+        losses = np.empty(n_iter, np.float32)
+        load().dcx_mlp_train(self.ptr, samples.ptr, n_iter, batch, lr, seed,
+                             fp(losses))
+        return losses
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            load().dcx_mlp_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Samples:
+    """Training samples on the device (dcx_samples_*)."""
+
+    def __init__(self, capacity: int, device=0):
+        self.ptr = load().dcx_samples_create(device, capacity)
+
+    def count(self) -> int:
+        return load().dcx_samples_count(self.ptr)
+
+    def clear(self):
+        load().dcx_samples_clear(self.ptr)
+
+    def upload(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        load().dcx_samples_upload(self.ptr, x.shape[0], fp(x), fp(t))
+
+    def download(self, first=0, n=None):
+        n = self.count() - first if n is None else n
+        x = np.empty((n, 64), np.float32)
+        t = np.empty((n, 2), np.float32)
+        s = np.empty(n, np.float32)
+        c = np.empty(n, np.uint32)
+        load().dcx_samples_download(self.ptr, first, n, fp(x), fp(t), fp(s),
+                                    c.ctypes.data_as(C.POINTER(C.c_uint)))
+        return x, t, s, c
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            load().dcx_samples_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_learned_layer(d: DeviceData, mlp, width: int) -> int:
+    """Attach (or, with mlp None, detach) the learned layer; returns the
+    number of layer cells."""
+    return load().dcx_set_learned_layer(d.ptr, None if mlp is None else mlp.ptr,
+                                        width)
+
+
+def learned_sample_step(padded: DeviceData, small: DeviceData, origin, width,
+                        n_cells, seed, samples: Samples, src_index=-1,
+                        src_next=0.0, add_next=False, rollout=False,
+                        taper=False) -> int:
+    """dcx_learned_sample_step with padded's spacing and dt."""
+    flags = (1 if rollout else 0) | (2 if taper else 0)
+    return load().dcx_learned_sample_step(
+        padded.ptr, small.ptr, (C.c_longlong * 3)(*origin), width, n_cells,
+        seed & 0xffffffff, samples.ptr, padded.dx, padded.dy, padded.dz,
+        padded.dt, src_index, src_next, int(add_next), flags)
 
 
 def program_run(size, absorption, d, dt, time_max, topology=(1, 1, 1),

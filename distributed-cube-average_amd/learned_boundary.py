@@ -1,0 +1,371 @@
+"""Learned absorbing layer: training data from the propagator, training on the
+GPU, and the reflection score against a padded-domain judge.
+
+The network and its kernels live in the library (include/dc_hip.h, "learned
+absorbing layer"); this module drives them:
+
+* `training_case` builds a padded domain (a smooth random TTI model with the
+  reference's random-velocity shell far outside) and a small domain, a window
+  of it whose outer `width` cells form the learned layer;
+* `collect` steps the padded domain with a Ricker source and, every few
+  steps, lets dcx_learned_sample_step record, at random layer cells of the
+  small domain, the network input and the correction that makes the small
+  domain's step (which sees zeros beyond its edge) equal the padded one's;
+* `train` runs Adam on those samples with the library's MFMA forward and
+  backward kernels;
+* `reflection` runs one source in four ways -- the padded judge, the small
+  domain with the learned layer, the small domain with nothing at its edge,
+  and the small domain whose outer cells are the reference's random-velocity
+  shell (src/boundary.c:7-104) -- and scores each by the energy of its
+  difference from the judge at interior receivers.
+
+The weights that ship with the package (`learned_layer_w6.npz`) were made by
+`python -m distributed-cube-average_amd.learned_boundary train`.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import dc_hip as H
+from .velocity_model import STENCIL, cfl_dt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "learned_layer_w6.npz")
+NPARAM = 64 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def init_params(seed=0) -> np.ndarray:
+    """He-normal hidden layers, a small output layer, zero biases."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.standard_normal((64, 64)) * np.sqrt(2.0 / 64)
+    W2 = rng.standard_normal((64, 64)) * np.sqrt(2.0 / 64)
+    W3 = rng.standard_normal((2, 64)) * 0.01
+    return np.concatenate([W1.ravel(), np.zeros(64), W2.ravel(), np.zeros(64),
+                           W3.ravel(), np.zeros(2)]).astype(np.float32)
+
+
+def save_weights(path, params, meta: dict):
+    np.savez(path, params=np.asarray(params, np.float32),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_weights(path=WEIGHTS):
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].astype(np.float32), {k: z[k] for k in z.files if k != "params"}
+
+
+def _smooth_noise(rng, shape, coarse=10):
+    """Seeded noise, smooth over ~shape/coarse cells, in [0, 1]."""
+    from scipy.ndimage import zoom
+    c = rng.standard_normal((coarse,) * 3)
+    f = zoom(c, [s / coarse for s in shape], order=1, mode="nearest")
+    f = f[:shape[0], :shape[1], :shape[2]]
+    return ((f - f.min()) / (f.max() - f.min() + 1e-30)).astype(np.float32)
+
+
+def random_tti(rng, shape):
+    """Smooth random TTI model: vpz in [1500, 4500] m/s, vsv = 0.5 vpz;
+    epsilon U[0,0.3], delta U[0, min(eps, 0.2)], theta U[0, pi/3], phi
+    U[0, pi] per cell (the anisotropy of the other workloads)."""
+    vpz = (1500.0 + 3000.0 * _smooth_noise(rng, shape)).astype(np.float32)
+    vsv = (0.5 * vpz).astype(np.float32)
+    eps = rng.uniform(0.0, 0.3, shape).astype(np.float32)
+    dlt = (rng.uniform(0.0, 1.0, shape) * np.minimum(eps, 0.2)).astype(np.float32)
+    theta = rng.uniform(0.0, np.pi / 3, shape).astype(np.float32)
+    phi = rng.uniform(0.0, np.pi, shape).astype(np.float32)
+    return vpz, vsv, eps, dlt, theta, phi
+
+
+@dataclass
+class Case:
+    """A padded domain and the small domain cut from its middle."""
+    coef_pad: list
+    pad_sizes: tuple      # array sizes (sx, sy, sz)
+    model: tuple          # vpz, vsv, eps, dlt, theta, phi of the padded arrays
+    origin: tuple         # padded cell of small's compute (0, 0, 0)
+    inner: int            # small compute extent per axis
+    width: int
+    d: float
+    dt: float
+    src: tuple            # source cell in padded array coordinates
+    fcut: float
+
+    def small_coef(self):
+        o = self.origin
+        n = self.inner + 2 * STENCIL
+        sl = (slice(o[2] - STENCIL, o[2] - STENCIL + n),
+              slice(o[1] - STENCIL, o[1] - STENCIL + n),
+              slice(o[0] - STENCIL, o[0] - STENCIL + n))
+        return [np.ascontiguousarray(c[sl]) for c in self.coef_pad]
+
+    def sponge_coef(self, absorb, extra=0):
+        """The small domain grown by `extra` cells per side whose outer
+        `absorb` compute cells hold the reference's random-velocity shell."""
+        o = self.origin
+        n = self.inner + 2 * extra + 2 * STENCIL
+        lo = [o[a] - STENCIL - extra for a in range(3)]
+        sl = (slice(lo[2], lo[2] + n), slice(lo[1], lo[1] + n),
+              slice(lo[0], lo[0] + n))
+        vpz, vsv, eps, dlt, theta, phi = [np.ascontiguousarray(m[sl]) for m in self.model]
+        problem = (self.inner + 2 * extra - 2 * absorb,) * 3
+        H.random_boundary((n, n, n), (n, n, n), (0, 0, 0), problem, STENCIL,
+                          absorb, vpz, vsv, 0)
+        return H.precompute(vpz, vsv, eps, dlt, phi, theta)
+
+    def pad_index(self, x, y, z):
+        sx, sy, _ = self.pad_sizes
+        return int(x + y * sx + z * sx * sy)
+
+    def small_index(self, x, y, z):
+        """Linear index in the small domain's arrays of padded cell (x,y,z)."""
+        n = self.inner + 2 * STENCIL
+        o = [a - STENCIL for a in self.origin]
+        return int((x - o[0]) + (y - o[1]) * n + (z - o[2]) * n * n)
+
+
+def make_case(seed, inner=64, width=6, margin=48, d=20.0, cfl_frac=0.4,
+              hz=(10.0, 25.0), src_margin=10, pad_absorb=24, src_depth=None):
+    """Padded compute region inner + 2 * margin per axis (the reference's
+    random-velocity shell, pad_absorb cells, on its outer faces); the source
+    is a Ricker of random peak frequency at a random cell of the small
+    domain's interior (at least width + src_margin cells from its edge), or,
+    with src_depth, src_depth cells below the layer of the small domain's
+    low-z face, centred in x and y."""
+    rng = np.random.default_rng(seed)
+    n = inner + 2 * margin + 2 * STENCIL
+    shape = (n, n, n)
+    model = random_tti(rng, shape)
+    vpz, vsv = model[0].copy(), model[1].copy()
+    problem = (inner + 2 * margin - 2 * pad_absorb,) * 3
+    H.random_boundary((n, n, n), (n, n, n), (0, 0, 0), problem, STENCIL,
+                      pad_absorb, vpz, vsv, 0)
+    coef = H.precompute(vpz, vsv, model[2], model[3], model[5], model[4])
+    dt = float(np.float32(cfl_frac * cfl_dt(coef[6], d, d, d)))
+    o = STENCIL + margin
+    lo, hi = width + src_margin, inner - width - src_margin
+    src = tuple(int(o + rng.integers(lo, hi)) for _ in range(3))
+    fcut = 3.0 * float(rng.uniform(*hz))
+    if src_depth is not None:
+        src = (o + inner // 2, o + inner // 2, o + width + src_depth)
+    return Case(coef, (n, n, n), model, (o, o, o), inner, width, d, dt, src, fcut)
+
+
+def collect(samples: H.Samples, seeds, steps=700, first=60, every=4,
+            cells_per_step=1024, mlp=None, rollout=False, taper=False,
+            **case_kw):
+    """Training samples from the padded runs of the given model seeds.
+    rollout: the small domain runs on its own state beside the padded one
+    (with `mlp` attached when given) and the samples are its states with the
+    corrections that pull its steps back to the padded run's."""
+    t0 = time.time()
+    for k, seed in enumerate(seeds):
+        c = make_case(seed, **case_kw)
+        pad = H.DeviceData(c.coef_pad, None, c.d, c.d, c.d, c.dt)
+        pad.release_host()
+        small = H.DeviceData(c.small_coef(), None, c.d, c.d, c.d, c.dt)
+        small.release_host()
+        if rollout and mlp is not None:
+            H.set_learned_layer(small, mlp, c.width)
+        amp = H.ricker_table(c.dt, steps + 1, c.fcut)
+        si = c.pad_index(*c.src)
+        ss = c.small_index(*c.src)
+        pad.add_source(si, float(amp[0]))
+        if rollout:
+            small.add_source(ss, float(amp[0]))
+        for it in range(steps):
+            if it >= first and (it - first) % every == 0:
+                H.learned_sample_step(pad, small, c.origin, c.width,
+                                      cells_per_step, seed * 100003 + it,
+                                      samples, si, float(amp[it + 1]), True,
+                                      rollout=rollout, taper=taper)
+            else:
+                pad.step(si, float(amp[it + 1]), True)
+                if rollout:
+                    small.step(ss, float(amp[it + 1]), True)
+        pad.synchronize()
+        small.synchronize()
+        pad.close()
+        small.close()
+        if k % 8 == 7 or k == len(seeds) - 1:
+            log(f"[learned] collected {samples.count()} samples from "
+                f"{k + 1} models ({time.time() - t0:.0f}s)")
+
+
+def train(samples: H.Samples, iters=6000, batch=4096, lr=2e-3, seed=0,
+          params=None, chunk=500, mode=H.MLP_FREE, sigma_max=0.0):
+    """Adam with a step decay of the learning rate; returns (mlp, losses)."""
+    mlp = H.Mlp(init_params(seed) if params is None else params, mode=mode,
+                sigma_max=sigma_max)
+    losses = []
+    t0 = time.time()
+    done = 0
+    while done < iters:
+        n = min(chunk, iters - done)
+        rate = lr * (0.1 ** (done / iters * 2.0))
+        losses.append(mlp.train(samples, n, batch, rate, seed + done))
+        done += n
+        log(f"[learned] iter {done}: loss {float(np.mean(losses[-1][-50:])):.4e} "
+            f"(lr {rate:.2e}, {time.time() - t0:.0f}s)")
+    return mlp, np.concatenate(losses)
+
+
+def _receivers(c: Case, gap=4, stride=4):
+    """Interior receivers: cells `gap` inside the learned layer on the six
+    faces of the small domain's interior, every `stride` cells, in padded
+    array coordinates."""
+    o, n, w = c.origin, c.inner, c.width
+    a, b = w + gap, n - 1 - w - gap
+    pts = set()
+    rng = range(a, b + 1, stride)
+    for u in rng:
+        for v in rng:
+            for fixed in (a, b):
+                pts.add((fixed, u, v))
+                pts.add((u, fixed, v))
+                pts.add((u, v, fixed))
+    return np.array(sorted(pts), np.int64) + np.array(o, np.int64)
+
+
+def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
+    """Energy ratio of (run - judge) to judge at the interior receivers over
+    `steps` steps, for the learned layer, no layer and random-velocity
+    shells of the given widths (shells wider than the layer grow the domain
+    outward so that the interior, the receivers and the source stay put)."""
+    rec = _receivers(c)
+    amp = H.ricker_table(c.dt, steps, c.fcut)
+
+    def run(coef, lo):
+        """Run a domain whose array (0,0,0) is padded cell lo; record p at
+        the receivers."""
+        sz = coef[0].shape
+        sx, sy = sz[2], sz[1]
+        idx = (rec[:, 0] - lo[0]) + (rec[:, 1] - lo[1]) * sx + \
+            (rec[:, 2] - lo[2]) * sx * sy
+        s = c.src
+        si = (s[0] - lo[0]) + (s[1] - lo[1]) * sx + (s[2] - lo[2]) * sx * sy
+        return coef, idx, int(si)
+
+    out = {}
+    # Judge: the padded domain.
+    d = H.DeviceData(c.coef_pad, None, c.d, c.d, c.d, c.dt)
+    _, idx, si = run(c.coef_pad, (0, 0, 0))
+    judge, _ = d.run_traces(steps, si, amp, idx, graph=graph)
+    d.close()
+    ej = float(np.sum(judge.astype(np.float64) ** 2))
+    lo_small = tuple(o - STENCIL for o in c.origin)
+
+    def score(coef, lo, net):
+        dd = H.DeviceData(coef, None, c.d, c.d, c.d, c.dt)
+        if net is not None:
+            H.set_learned_layer(dd, net, c.width)
+        _, idx2, si2 = run(coef, lo)
+        tr, ms = dd.run_traces(steps, si2, amp, idx2, graph=graph)
+        n = coef[0].shape
+        log(f"[learned]   {'learned' if net is not None else 'plain'} "
+            f"{n[2]}x{n[1]}x{n[0]}: {ms / steps:.3f} ms/step")
+        dd.close()
+        if not np.all(np.isfinite(tr)):
+            return float("inf")
+        return float(np.sum((tr.astype(np.float64) - judge) ** 2) / ej)
+
+    small = c.small_coef()
+    out["learned"] = score(small, lo_small, mlp)
+    out["none"] = score(small, lo_small, None)
+    for w in sponge_widths:
+        extra = max(0, w - c.width)
+        coef = c.sponge_coef(w, extra)
+        lo = tuple(o - STENCIL - extra for o in c.origin)
+        out[f"sponge{w}"] = score(coef, lo, None)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--models", type=int, default=64)
+    ap.add_argument("--inner", type=int, default=64)
+    ap.add_argument("--margin", type=int, default=48)
+    ap.add_argument("--width", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=700)
+    ap.add_argument("--iters", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--eval-seeds", type=int, nargs="*", default=[9001, 9002])
+    ap.add_argument("--eval-steps", type=int, default=900)
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--mode", choices=["free", "damping"], default="damping")
+    ap.add_argument("--sigma-max", type=float, default=0.5)
+    ap.add_argument("--rounds", type=int, default=3,
+                    help="closed-loop collection rounds after the first")
+    ap.add_argument("--round-iters", type=int, default=3000)
+    ap.add_argument("--eval-margin", type=int, default=96)
+    ap.add_argument("--taper", type=int, default=1)
+    ap.add_argument("--src-depth", type=int, default=None,
+                    help="evaluation source this many cells below the low-z layer")
+    args = ap.parse_args(argv)
+    kw = dict(inner=args.inner, width=args.width, margin=args.margin)
+    ekw = dict(inner=args.inner, width=args.width, margin=args.eval_margin,
+               src_depth=args.src_depth)
+    mode = H.MLP_DAMPING if args.mode == "damping" else H.MLP_FREE
+    sig = args.sigma_max if mode == H.MLP_DAMPING else 0.0
+
+    def evaluate(mlp, tag):
+        scores = {}
+        for s in args.eval_seeds:
+            r = reflection(mlp, make_case(s, **ekw), args.eval_steps)
+            log(f"[learned] {tag} eval seed {s}: " +
+                ", ".join(f"{k} {v:.3e}" for k, v in r.items()))
+            for k, v in r.items():
+                scores.setdefault(k, []).append(v)
+        return scores
+
+    if args.cmd == "train":
+        per_model = ((args.steps - 60 + 3) // 4) * args.cells
+        samples = H.Samples(per_model * args.models * (1 + args.rounds) + 1)
+        seeds = list(range(args.models))
+        collect(samples, seeds, steps=args.steps, cells_per_step=args.cells,
+                taper=bool(args.taper), **kw)
+        mlp, losses = train(samples, iters=args.iters, batch=args.batch,
+                            lr=args.lr, mode=mode, sigma_max=sig)
+        scores = evaluate(mlp, "teacher-forced")
+        for r in range(args.rounds):
+            # Closed loop: the small domain runs with the current network.
+            collect(samples, [1000 * (r + 1) + k for k in seeds],
+                    steps=args.steps, cells_per_step=args.cells, mlp=mlp,
+                    rollout=True, taper=bool(args.taper), **kw)
+            mlp, losses = train(samples, iters=args.round_iters,
+                                batch=args.batch, lr=args.lr * 0.5,
+                                seed=r + 1, params=mlp.params(), mode=mode,
+                                sigma_max=sig)
+            scores = evaluate(mlp, f"round {r + 1}")
+        params = mlp.params()
+        save_weights(args.out, params, dict(
+            width=args.width, inner=args.inner, margin=args.margin,
+            models=args.models, steps=args.steps, iters=args.iters,
+            rounds=args.rounds, round_iters=args.round_iters,
+            batch=args.batch, lr=args.lr, samples=samples.count(),
+            mode=mode, sigma_max=sig, eval_margin=args.eval_margin,
+            taper=args.taper,
+            eval_steps=args.eval_steps, eval_seeds=np.array(args.eval_seeds),
+            final_loss=float(np.mean(losses[-100:])),
+            **{f"eval_{k}": np.array(v) for k, v in scores.items()}))
+        log(f"[learned] wrote {args.out}")
+    else:
+        params, meta = load_weights(args.out)
+        mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
+        evaluate(mlp, "shipped")
+
+
+if __name__ == "__main__":
+    main()

@@ -203,6 +203,116 @@ double dcx_batch_run(dcx_batch *batch, unsigned int n_steps, const float *d4,
                      const long long *src_index, const float *src_amp,
                      int use_graph);
 
+/* ------------------------------------------- learned absorbing layer -- */
+
+/* A small network on every cell of a thin layer (`width` cells, 1..8) along
+ * the six faces of the compute region that corrects the new p and q the
+ * stencil writes there, in place of a thick absorbing shell.  No reference
+ * counterpart: the reference absorbs with its random-velocity shell
+ * (src/boundary.c:7-104), which stays the fallback (set no network).
+ *
+ * Input of a layer cell (64 values): the 3x3x3 patch of the time
+ * differences p^n - p^{n-1} and q^n - q^{n-1} around it (54 values), taken
+ * in the frame of its nearest face (axis order: outward normal, then the two
+ * other axes in increasing order; p's 27 then q's 27, the last axis
+ * fastest) and divided by their largest magnitude s; a one-hot of the
+ * cell's depth in the layer (inputs 54..61); the cell's Courant numbers
+ * sqrt(v2px) * c and sqrt(v2pz) * c with c = dt * sqrt(1/dx^2) (inputs 62,
+ * 63; 1/dx^2 formed in float).  Network: 64 -> 64 (ReLU) ->
+ * 64 (ReLU) -> 2, FP16 weights and activations on MFMA, FP32 accumulation.
+ * Correction of the new p and q of the cell, by output form
+ * (dcx_mlp_set_output):
+ *   DCX_MLP_FREE     (dp, dq) = s * y (0 where s = 0);
+ *   DCX_MLP_DAMPING  dp = -sigma_max * sigmoid(y0) * (p^n - p^{n-1}) at the
+ *                    cell (dq likewise with y1): a learned damping that can
+ *                    only take energy out (0 < sigma_max < 1).
+ *
+ * Parameters (dcx_mlp_nparam() floats): W1 [64][64] (out, in), b1 [64],
+ * W2 [64][64], b2 [64], W3 [2][64], b3 [2]. */
+typedef struct dcx_mlp dcx_mlp;
+typedef struct dcx_samples dcx_samples;
+
+enum { DCX_MLP_FREE = 0, DCX_MLP_DAMPING = 1 };
+
+int dcx_mlp_nparam(void);
+/* A network with zero parameters and the free output form. */
+dcx_mlp *dcx_mlp_create(int device);
+/* Output form and sigma_max (DCX_MLP_DAMPING: 0 < sigma_max < 1); the
+ * training loss compares the prediction of that form, divided by s, with
+ * the targets.  Returns 0, or -1 (nothing changed). */
+int dcx_mlp_set_output(dcx_mlp *mlp, int mode, float sigma_max);
+void dcx_mlp_free(dcx_mlp *mlp);
+/* Set the parameters (and reset the optimiser state) / read them back. */
+void dcx_mlp_set_params(dcx_mlp *mlp, const float *params);
+void dcx_mlp_get_params(const dcx_mlp *mlp, float *params);
+
+/* Training samples on the device: inputs (stored as FP16), targets (the
+ * correction divided by s) and s. */
+dcx_samples *dcx_samples_create(int device, long long capacity);
+void dcx_samples_free(dcx_samples *samples);
+long long dcx_samples_count(const dcx_samples *samples);
+void dcx_samples_clear(dcx_samples *samples);
+/* Append n samples given as FP32 inputs [n][64] and targets [n][2] (s = 1). */
+void dcx_samples_upload(dcx_samples *samples, long long n, const float *x,
+                        const float *t);
+/* Copy samples [first, first + n) to the host: inputs [n][64], targets
+ * [n][2], s, and the cell (x | y << 10 | z << 20 in compute coordinates of
+ * the small struct, 0 for uploaded samples); any pointer may be NULL. */
+void dcx_samples_download(const dcx_samples *samples, long long first,
+                          long long n, float *x, float *t, float *scale,
+                          unsigned int *codes);
+
+/* Raw network outputs y of every stored sample (before the output form):
+ * y [count][2]. */
+void dcx_mlp_forward(dcx_mlp *mlp, const dcx_samples *samples, float *y);
+/* Gradient of the mean over all stored samples of (u0 - t0)^2 + (u1 - t1)^2
+ * with respect to the parameters, without a step, where u is the output
+ * form's correction divided by s (free: u = y; damping: u = -sigma_max *
+ * sigmoid(y) * x_c with x_c the cell's own normalised time difference,
+ * input 13 for p, 40 for q); returns that mean loss. */
+double dcx_mlp_gradient(dcx_mlp *mlp, const dcx_samples *samples, float *grad);
+/* n_iter Adam steps (beta 0.9/0.999, eps 1e-8) on minibatches of `batch`
+ * samples drawn uniformly with replacement (seeded); loss_per_iter (may be
+ * NULL) receives each minibatch's mean loss.  Returns the mean over the
+ * iterations. */
+double dcx_mlp_train(dcx_mlp *mlp, const dcx_samples *samples, int n_iter,
+                     int batch, float lr, long long seed, float *loss_per_iter);
+
+/* Attach the network to a struct (FP32 storage): every dcx_step / dcx_run
+ * step then corrects its layer cells.  NULL or width 0 detaches it.  Returns
+ * the number of layer cells. */
+int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *mlp, int width);
+
+/* One training-data step.  `small` is a window of `padded`: its compute
+ * region starts at padded cell origin[3] (>= 4 per axis).
+ * flags: DCX_SAMPLE_ROLLOUT and/or DCX_SAMPLE_TAPER.
+ * Without DCX_SAMPLE_ROLLOUT (teacher forcing):
+ *  1. copies the padded fields over small's compute region (small's 4-cell
+ *     frame stays zero: small has no cells beyond its edge),
+ *  2. stores the inputs of n_cells random cells of small's layer,
+ *  3. steps small once without a source and without swapping;
+ * with DCX_SAMPLE_TAPER the copied layer cells are first multiplied by the
+ * taper a(d) = sin^2(pi (d + 1) / (2 (width + 1))) of their depth d.
+ * With DCX_SAMPLE_ROLLOUT (small runs on its own state, e.g. with the
+ * network attached):
+ *  2. stores the inputs of n_cells random layer cells of small's fields,
+ *  3. runs small's whole step (dcx_step, with padded's source moved into its
+ *     coordinates when inside), noting its stencil's new p, q before the
+ *     learned corrections;
+ * then in both:
+ *  4. runs dcx_step on padded (with its source arguments),
+ *  5. stores target = (a * padded's new p, q - small's stencil's new p, q) / s
+ * at those cells (a = the taper with DCX_SAMPLE_TAPER, else 1): the
+ * correction that makes small's step match padded's (tapered) one.
+ * Returns the new sample count. */
+enum { DCX_SAMPLE_ROLLOUT = 1, DCX_SAMPLE_TAPER = 2 };
+long long dcx_learned_sample_step(dc_device_data *padded, dc_device_data *small,
+                                  const long long origin[3], int width,
+                                  int n_cells, unsigned int seed,
+                                  dcx_samples *samples, float dx, float dy,
+                                  float dz, float dt, long long src_index,
+                                  float src_next, int add_next, int flags);
+
 /* ---------------------------------------------- whole program, 1 GPU ---- */
 
 /* The reference program (src/main.c) on one GPU: global grid
