@@ -44,8 +44,21 @@ constexpr int oW1 = 0, oB1 = oW1 + kHid * kIn, oW2 = oB1 + kHid,
               oB3 = oW3 + kOut * kHid, kNParam = oB3 + kOut;
 // Packed fp16 operands: W1 [64][64], W2 [64][64], W3 [16][64] (rows >= 2
 // zero), W2^T [64][64], W3^T [64][16] (columns >= 2 zero).
+// W1p [64][64] is W1 with its columns in the layer inference's slot order
+// (see slot_feature).
 constexpr int hW1 = 0, hW2 = hW1 + 4096, hW3 = hW2 + 4096, hW2t = hW3 + 1024,
-              hW3t = hW2t + 4096, kNHalf = hW3t + 1024;
+              hW3t = hW2t + 4096, hW1p = hW3t + 1024, kNHalf = hW1p + 4096;
+
+// Layer inference reads its inputs in slot order, chosen so that K steps 0-1
+// (slots 0..31) touch only the p arrays and K steps 2-3 only the q arrays:
+// slots 0..26 p patch, 27..31 inputs 54..58, 32..58 q patch, 59..63 inputs
+// 59..63.  slot_feature maps a slot to its input, feature_slot back.
+__host__ __device__ constexpr int slot_feature(int s) {
+  return s < 27 ? s : s < 32 ? s + 27 : s < 59 ? s - 5 : s;
+}
+__host__ __device__ constexpr int feature_slot(int k) {
+  return k < 27 ? k : k < 54 ? k + 5 : k < 59 ? k - 27 : k;
+}
 
 enum { kOutFree = 0, kOutDamping = 1 };
 
@@ -236,34 +249,187 @@ __device__ inline float cell_inputs(h4 X[4], bool valid, uint32_t code,
   return m;
 }
 
+// What slot s of the layer inference holds: bit 0 set for a patch value,
+// then (i+1, j+1, k+1) two bits each from bit 1; otherwise the input index
+// (54..63) from bit 1.
+__device__ inline int slot_desc(int s) {
+  const int k = slot_feature(s);
+  if (k < kPatch) {
+    const int idx = k < 27 ? k : k - 27;
+    return 1 | ((idx / 9) << 1) | (((idx / 3) % 3) << 3) | ((idx % 3) << 5);
+  }
+  return k << 1;
+}
+
+// The loads of one group of 16 cells for one lane, kept apart from their use
+// so that the next group's loads fly while the current group runs on MFMA.
+// Offsets are 32-bit (a layer domain's arrays hold fewer than 2^31 floats)
+// and every K step reads one pair of arrays, so each load is a uniform base
+// plus a per-lane offset.
+struct GroupLoads {
+  float a[4][4], b[4][4];
+  float vx, vz;
+  int own, depth;
+};
+
+__device__ inline void group_load(GroupLoads &L, const int desc[4][4],
+                                  bool valid, uint32_t code, const LayerGeom &g,
+                                  const float *pc, const float *pp,
+                                  const float *qc, const float *qp) {
+  int own = 0, sn = 0, s1 = 0, s2 = 0, depth = 0;
+  if (valid) {
+    const CellFrame f = cell_frame(code, g);
+    own = (int)f.own;
+    sn = (int)f.sn;
+    s1 = (int)f.s1;
+    s2 = (int)f.s2;
+    depth = f.depth;
+  }
+  L.own = own;
+  L.depth = depth;
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const float *cur = ks < 2 ? pc : qc;
+    const float *prev = ks < 2 ? pp : qp;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int d = desc[ks][r];
+      const int o = (d & 1) ? own + (((d >> 1) & 3) - 1) * sn +
+                                  (((d >> 3) & 3) - 1) * s1 + (((d >> 5) & 3) - 1) * s2
+                            : own;
+      L.a[ks][r] = cur[(uint32_t)o];
+      L.b[ks][r] = prev[(uint32_t)o];
+    }
+  }
+  L.vx = g.v2px[(uint32_t)own];
+  L.vz = g.v2pz[(uint32_t)own];
+}
+
 // Corrections of the layer cells from the current fields (before the step).
+// Weights and biases are staged in LDS (the A operands are read per MFMA) to
+// leave registers for the loads in flight; W1 is the slot-ordered copy.
 __global__ void __launch_bounds__(256)
     k_layer_infer(const float *pc, const float *pp, const float *qc,
                   const float *qp, LayerGeom g, const uint32_t *__restrict__ cells,
                   int ncell, const _Float16 *__restrict__ wh,
                   const float *__restrict__ w32, float2 *__restrict__ delta,
                   int mode, float sigma_max) {
+  constexpr int lW1 = 0, lW2 = 1024, lW3 = 2048;  // h4 offsets in sW
+  __shared__ h4 sW[2048 + 256];
+  __shared__ float sB[kHid + kHid + 16];
+  __shared__ int sDesc[kIn];
+  for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
+    sW[lW1 + t] = reinterpret_cast<const h4 *>(wh + hW1p)[t];
+    sW[lW2 + t] = reinterpret_cast<const h4 *>(wh + hW2)[t];
+  }
+  for (int t = threadIdx.x; t < 256; t += blockDim.x)
+    sW[lW3 + t] = reinterpret_cast<const h4 *>(wh + hW3)[t];
+  for (int t = threadIdx.x; t < kHid; t += blockDim.x) {
+    sB[t] = w32[oB1 + t];
+    sB[kHid + t] = w32[oB2 + t];
+  }
+  if (threadIdx.x < 16)
+    sB[2 * kHid + threadIdx.x] = threadIdx.x < kOut ? w32[oB3 + threadIdx.x] : 0.0f;
+  if (threadIdx.x < kIn) sDesc[threadIdx.x] = slot_desc(threadIdx.x);
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   const int nwave = gridDim.x * (blockDim.x / 64);
-  FwdWeights w;
-  load_fwd(w, wh, w32, lane);
-  const int col = lane & 15;
-  for (int g0 = wave * 16; g0 < ncell; g0 += nwave * 16) {
+  const int col = lane & 15, kb = lane >> 4;
+  int desc[4][4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) desc[ks][r] = sDesc[16 * ks + 4 * kb + r];
+  const int stride = nwave * 16;
+  int g0 = wave * 16;
+  GroupLoads ld;
+  {
+    const int c = g0 + col;
+    group_load(ld, desc, c < ncell, c < ncell ? cells[c] : 0u, g, pc, pp, qc, qp);
+  }
+  for (; g0 < ncell; g0 += stride) {
     const int c = g0 + col;
     const bool valid = c < ncell;
+    // An offset the compiler cannot see is zero: it keeps the LDS operand
+    // reads inside the loop instead of holding all 36 in registers.
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    // Consume this group's loads (the same arithmetic as raw_feature and
+    // norm_feature), then start the next group's.
+    float v[4][4];
+    float m = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int d = desc[ks][r];
+        float x;
+        if (d & 1) {
+          x = ld.a[ks][r] - ld.b[ks][r];
+          m = fmaxf(m, fabsf(x));
+        } else {
+          const int k = d >> 1;
+          x = k < kCourant ? ((k - kPatch) == ld.depth ? 1.0f : 0.0f)
+                           : sqrtf(k == kCourant ? ld.vx : ld.vz) * g.cdt;
+        }
+        v[ks][r] = valid ? x : 0.0f;
+      }
+    const int own = ld.own;
+    {
+      const int cn = g0 + stride + col;
+      const bool vn = cn < ncell;
+      group_load(ld, desc, vn, vn ? cells[cn] : 0u, g, pc, pp, qc, qp);
+    }
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
     h4 X[4];
-    int64_t own = 0;
-    const float s = cell_inputs(X, valid, valid ? cells[c] : 0u, g, pc, pp, qc,
-                                qp, lane, &own);
-    const f4 y = mlp_forward(w, X);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float x = v[ks][r];
+        X[ks][r] = (_Float16)((desc[ks][r] & 1) ? (m > 0.0f ? x / m : 0.0f) : x);
+      }
+    h4 H1[4], H2[4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      f4 acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = sB[z + 16 * ot + 4 * kb + r];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
+            sW[z + lW1 + ((16 * ot + col) * 64 + 16 * ks + 4 * kb) / 4], X[ks], acc, 0, 0, 0);
+      H1[ot] = relu_h(acc);
+    }
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      f4 acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = sB[z + kHid + 16 * ot + 4 * kb + r];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
+            sW[z + lW2 + ((16 * ot + col) * 64 + 16 * ks + 4 * kb) / 4], H1[ks], acc, 0, 0, 0);
+      H2[ot] = relu_h(acc);
+    }
+    f4 y;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] = sB[z + 2 * kHid + 4 * kb + r];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      y = __builtin_amdgcn_mfma_f32_16x16x16f16(
+          sW[z + lW3 + (col * 64 + 16 * ks + 4 * kb) / 4], H2[ks], y, 0, 0, 0);
     if (lane < 16 && valid) {
       if (mode == kOutDamping) {
+        // The cell's own time differences (inputs 13 and 40 are held by
+        // other lanes): read again, from cache.
         const float sp = sigma_max / (1.0f + expf(-y[0]));
         const float sq = sigma_max / (1.0f + expf(-y[1]));
         delta[c] = make_float2(-sp * (pc[own] - pp[own]), -sq * (qc[own] - qp[own]));
       } else {
-        delta[c] = make_float2(s * y[0], s * y[1]);
+        delta[c] = make_float2(m * y[0], m * y[1]);
       }
     }
   }
@@ -547,6 +713,7 @@ __global__ void k_mlp_update(float *w32, float *m, float *v, float *grad,
   if (p < oB1) {
     const int o = p / 64, i = p % 64;
     wh[hW1 + o * 64 + i] = h;
+    wh[hW1p + o * 64 + feature_slot(i)] = h;
   } else if (p >= oW2 && p < oB2) {
     const int q = p - oW2, o = q / 64, i = q % 64;
     wh[hW2 + o * 64 + i] = h;

@@ -289,9 +289,34 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
     return out
 
 
+def cost(n=384, width=6, steps=50, seed=0):
+    """Device ms per step of an n^3-compute domain (random smooth TTI model)
+    without and with the learned layer: (plain, learned, layer cells)."""
+    rng = np.random.default_rng(seed)
+    shape = (n + 2 * STENCIL,) * 3
+    vpz, vsv, eps, dlt, theta, phi = random_tti(rng, shape)
+    coef = H.precompute(vpz, vsv, eps, dlt, phi, theta)
+    del vpz, vsv, eps, dlt, theta, phi
+    dt = float(np.float32(0.4 * cfl_dt(coef[6], 20.0, 20.0, 20.0)))
+    d = H.DeviceData(coef, None, 20.0, 20.0, 20.0, dt)
+    d.release_host()
+    src = (n // 2 + STENCIL) * (1 + shape[0] + shape[0] ** 2)
+    amp = H.ricker_table(dt, steps, 45.0)
+    d.run(5, src, amp[:5], graph=True)
+    plain = d.run(steps, src, amp, graph=True) / steps
+    params, meta = load_weights()
+    mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
+    ncell = H.set_learned_layer(d, mlp, width)
+    d.run(5, src, amp[:5], graph=True)
+    learned = d.run(steps, src, amp, graph=True) / steps
+    d.close()
+    mlp.close()
+    return plain, learned, ncell
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "cost"])
     ap.add_argument("--models", type=int, default=64)
     ap.add_argument("--inner", type=int, default=64)
     ap.add_argument("--margin", type=int, default=48)
@@ -330,6 +355,12 @@ def main(argv=None):
                 scores.setdefault(k, []).append(v)
         return scores
 
+    if args.cmd == "cost":
+        plain, learned, ncell = cost(args.inner, args.width)
+        log(f"[learned] cost {args.inner}^3: plain {plain:.3f} ms/step, with the "
+            f"{args.width}-cell layer ({ncell} cells) {learned:.3f} ms/step; layer "
+            f"{learned - plain:.3f} ms = {ncell / (learned - plain) / 1e6:.2f} Gcell/s")
+        return
     if args.cmd == "train":
         per_model = ((args.steps - 60 + 3) // 4) * args.cells
         samples = H.Samples(per_model * args.models * (1 + args.rounds) + 1)
