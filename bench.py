@@ -364,6 +364,41 @@ def measure_traffic(args):
     return res
 
 
+def copy_bandwidth(gib=1.0, reps=20):
+    """HIP runtime device-to-device copy rate (GB/s, read + write) of a
+    buffer four times the 256 MB last-level cache, timed with HIP events:
+    what a plain streaming copy gets from this GPU's HBM, reported beside the
+    published peak."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(gib * (1 << 30))
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ms = C.c_float()
+    try:
+        for call in (hip.hipMalloc(C.byref(a), C.c_size_t(n)),
+                     hip.hipMalloc(C.byref(b), C.c_size_t(n)),
+                     hip.hipEventCreate(C.byref(e0)),
+                     hip.hipEventCreate(C.byref(e1))):
+            if call != 0:
+                raise RuntimeError(f"HIP error {call}")
+        hip.hipMemcpyAsync(b, a, C.c_size_t(n), 3, None)  # device to device
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, C.c_size_t(n), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    finally:
+        for p in (a, b):
+            if p.value:
+                hip.hipFree(p)
+        for e in (e0, e1):
+            if e.value:
+                hip.hipEventDestroy(e)
+    return 2.0 * n * reps / (ms.value / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,7 +409,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 512))
     ap.add_argument("--xcd-remap", type=int, default=0)
-    ap.add_argument("--prefetch", type=int, default=2, choices=(1, 2))
+    ap.add_argument("--prefetch", type=int, default=2, choices=(1, 2, 3))
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=100)
@@ -439,6 +474,8 @@ def main():
     if not args.no_forms:
         forms = time_forms(H, w, d, args, amp, cells)
     fields = d.fields()
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device-to-device copy of 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
     d.close()
     batch = None
     if not args.no_batch:
@@ -461,6 +498,9 @@ def main():
             "algorithmic_bytes": alg_bytes}
     if traffic is not None:
         roof["traffic_detail"] = {k: round(v, 4) for k, v in traffic.items()}
+    if copy_gbs is not None:
+        roof["copy_gbs"] = round(copy_gbs, 1)
+        roof["frac_of_copy"] = round(achieved / copy_gbs, 4)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

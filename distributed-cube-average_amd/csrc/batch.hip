@@ -76,11 +76,11 @@ void dcx_batch_free(dcx_batch *bb) {
 
 int dcx_batch_set_tiling(dcx_batch *bb, int z_chunk, int prefetch_planes,
                          int blocks_per_cu) {
-  if (bb == nullptr || z_chunk < 1 || (prefetch_planes != 1 && prefetch_planes != 2) ||
+  if (bb == nullptr || z_chunk < 1 || prefetch_planes < 1 || prefetch_planes > 3 ||
       blocks_per_cu < 0 || blocks_per_cu > 64)
     return -1;
   bb->b.zchunk = z_chunk;
-  bb->b.deep = prefetch_planes == 2;
+  bb->b.deep = prefetch_planes - 1;
   bb->b.blocks_per_cu = blocks_per_cu;
   return 0;
 }
@@ -94,7 +94,7 @@ int dcx_batch_set_autotune(dcx_batch *bb, int enabled) {
 void dcx_batch_get_tiling(const dcx_batch *bb, int out[3]) {
   if (bb == nullptr) DCX_FAIL("null batch");
   out[0] = bb->b.zchunk;
-  out[1] = bb->b.deep ? 2 : 1;
+  out[1] = bb->b.deep + 1;
   out[2] = bb->b.blocks_per_cu;
 }
 
@@ -268,6 +268,8 @@ double dcx_batch_run(dcx_batch *bb, unsigned int n_steps, const float *d4,
     for (int zc : {32, 64, 128, 256, 512}) trial(zc, 1, 1);
     const int zc = best[0];
     trial(zc, 0, 1);
+    if (b.shots[0]->fstore == DCX_STORE_F32 && b.shots[0]->cstore == DCX_COEF_F32)
+      trial(zc, 2, 1);
     const int dp = best[1];
     for (int bpc : {2, 4, 0}) trial(zc, dp, bpc);
     b.zchunk = best[0];

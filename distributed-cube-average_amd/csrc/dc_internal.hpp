@@ -75,7 +75,7 @@ struct StepArgs {
   int fstore, cstore;              // DCX_STORE_*, DCX_COEF_*
   int zchunk;     // planes per workgroup (fast kernel)
   int xcd_remap;  // give each XCD a contiguous run of tiles (fast kernel)
-  int deep;       // two-plane prefetch variant (fast kernel)
+  int deep;       // fast kernel prefetch depth - 1 (0, 1, 2)
   int gx, gy;     // tiles along x and y (set by the launcher)
 };
 

@@ -486,9 +486,9 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk) {
 
 int dcx_set_prefetch_depth(dc_device_data *data, int planes) {
   State *st = state_of(data);
-  const int prev = st->deep ? 2 : 1;
-  if (planes != 1 && planes != 2) DCX_FAIL("prefetch depth must be 1 or 2");
-  st->deep = planes == 2;
+  const int prev = st->deep + 1;
+  if (planes < 1 || planes > 3) DCX_FAIL("prefetch depth must be 1, 2 or 3");
+  st->deep = planes - 1;
   return prev;
 }
 

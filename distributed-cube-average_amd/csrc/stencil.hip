@@ -251,12 +251,13 @@ struct Staged {
   float cp, cq, ap, aq;
 };
 
-// DEEP = false: the staged plane is fetched one plane ahead and the step's
-// coefficients at the top of the step.  DEEP = true: the staged plane two
+// PF = 1: the staged plane is fetched one plane ahead and the step's
+// coefficients at the top of the step.  PF = 2: the staged plane two
 // planes ahead and the coefficients one step ahead, doubling the bytes each
-// wave keeps in flight (more VGPRs; same occupancy at 512 threads).
+// wave keeps in flight (more VGPRs; same occupancy at 512 threads).  PF = 3:
+// the coefficients and old fields two steps ahead.
 // FS/CS select the field and coefficient storage.
-template <int TX, int TY, bool DEEP, int FS, int CS>
+template <int TX, int TY, int PF, int FS, int CS>
 __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
                                           const float src_amp) {
   using F = FieldIO<FS>;
@@ -390,11 +391,12 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 
   Staged cur = load_staged(zb - 4);
   Staged nx1 = cur;
-  StepLoads lcur{};
-  if (DEEP) {
+  StepLoads lcur{}, lnx{};
+  if (PF >= 2) {
     nx1 = load_staged(zb - 3);
     lcur = load_step(zb - 4);
   }
+  if (PF >= 3) lnx = load_step(zb - 3);
 
   for (int nb = zb - 4; nb < nend; nb += 8) {
 #pragma unroll
@@ -412,7 +414,13 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 
       StepLoads L;
       Staged nxt;
-      if (DEEP) {
+      if (PF >= 3) {
+        L = lcur;
+        nxt = nx1;
+        nx1 = load_staged(n + 2);
+        lcur = lnx;
+        lnx = load_step(n + 2);
+      } else if (PF == 2) {
         L = lcur;
         nxt = nx1;
         nx1 = load_staged(n + 2);
@@ -531,7 +539,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   }
 }
 
-template <int TX, int TY, bool DEEP, int FS, int CS>
+template <int TX, int TY, int PF, int FS, int CS>
 __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   // Workgroups are dealt to the 8 XCDs round-robin by linear id; remapping
   // gives each XCD (and its L2) a contiguous run of tiles, x fastest, so the
@@ -539,13 +547,13 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   int b = blockIdx.x;
   const int G = gridDim.x;
   if (a.xcd_remap && G % 8 == 0) b = (b % 8) * (G / 8) + b / 8;
-  fast_tile<TX, TY, DEEP, FS, CS>(a, b, a.src_amp);
+  fast_tile<TX, TY, PF, FS, CS>(a, b, a.src_amp);
 }
 
 // Persistent launch over the tiles of many volumes (shots): workgroup g takes
 // work items g, g + G, g + 2G, ... of the concatenated tile lists; every
 // workgroup runs the same number of items' worth of barriers on its own.
-template <int TX, int TY, bool DEEP, int FS, int CS>
+template <int TX, int TY, int PF, int FS, int CS>
 __global__ void __launch_bounds__(TX *TY)
     k_fast_batch(const StepArgs *__restrict__ shots,
                  const int64_t *__restrict__ prefix, int nshots,
@@ -564,7 +572,7 @@ __global__ void __launch_bounds__(TX *TY)
     const float amp = (a.src_z >= 0 && a.src_tab != nullptr && it + 1 < n_steps)
                           ? a.src_tab[it + 1]
                           : 0.0f;
-    fast_tile<TX, TY, DEEP, FS, CS>(a, (int)(w - prefix[k]), amp);
+    fast_tile<TX, TY, PF, FS, CS>(a, (int)(w - prefix[k]), amp);
   }
 }
 
@@ -576,10 +584,18 @@ bool fast_tiling_supported(int tx, int ty) {
 template <int TX, int TY, int FS, int CS>
 static void launch_fast_t(const StepArgs &b, dim3 grid, dim3 block,
                           hipStream_t s) {
+  // Three-level prefetch is built for 32x16 FP32 only (it needs the
+  // registers the other forms spend on decoding); they take two levels.
+  if constexpr (TX == 32 && TY == 16 && FS == DCX_STORE_F32 && CS == DCX_COEF_F32) {
+    if (b.deep >= 2) {
+      hipLaunchKernelGGL((k_fast<TX, TY, 3, FS, CS>), grid, block, 0, s, b);
+      return;
+    }
+  }
   if (b.deep)
-    hipLaunchKernelGGL((k_fast<TX, TY, true, FS, CS>), grid, block, 0, s, b);
+    hipLaunchKernelGGL((k_fast<TX, TY, 2, FS, CS>), grid, block, 0, s, b);
   else
-    hipLaunchKernelGGL((k_fast<TX, TY, false, FS, CS>), grid, block, 0, s, b);
+    hipLaunchKernelGGL((k_fast<TX, TY, 1, FS, CS>), grid, block, 0, s, b);
 }
 
 template <int TX, int TY>
@@ -612,12 +628,20 @@ template <int TX, int TY, int FS, int CS>
 static void launch_batch_t(const StepArgs *shots, const int64_t *prefix,
                            int nshots, int64_t total, int deep, dim3 grid,
                            int it, int n_steps, hipStream_t s) {
+  if constexpr (FS == DCX_STORE_F32 && CS == DCX_COEF_F32) {
+    if (deep >= 2) {
+      hipLaunchKernelGGL((k_fast_batch<TX, TY, 3, FS, CS>), grid,
+                         dim3(TX * TY), 0, s, shots, prefix, nshots, total, it,
+                         n_steps);
+      return;
+    }
+  }
   if (deep)
-    hipLaunchKernelGGL((k_fast_batch<TX, TY, true, FS, CS>), grid,
+    hipLaunchKernelGGL((k_fast_batch<TX, TY, 2, FS, CS>), grid,
                        dim3(TX * TY), 0, s, shots, prefix, nshots, total, it,
                        n_steps);
   else
-    hipLaunchKernelGGL((k_fast_batch<TX, TY, false, FS, CS>), grid,
+    hipLaunchKernelGGL((k_fast_batch<TX, TY, 1, FS, CS>), grid,
                        dim3(TX * TY), 0, s, shots, prefix, nshots, total, it,
                        n_steps);
 }

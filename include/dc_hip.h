@@ -43,7 +43,9 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
 
 /* Fast kernel prefetch depth: 1 = the staged plane one plane ahead and the
  * coefficients at the top of their step; 2 = the staged plane two planes
- * ahead and the coefficients one step ahead.  Returns the previous depth. */
+ * ahead and the coefficients one step ahead; 3 = as 2 with the coefficients
+ * and old fields two steps ahead (32x16 FP32 only; other forms use 2).
+ * Returns the previous depth. */
 int dcx_set_prefetch_depth(dc_device_data *data, int planes);
 
 /* Fast kernel: deal tiles to the GPU's 8 XCDs in contiguous runs so that
@@ -173,7 +175,7 @@ typedef struct dcx_batch dcx_batch;
 dcx_batch *dcx_batch_create(dc_device_data **shots, int n);
 void dcx_batch_free(dcx_batch *batch);
 
-/* Z chunk of the 32x16-cell tiles, prefetch depth (1 or 2 planes) and
+/* Z chunk of the 32x16-cell tiles, prefetch depth (1, 2 or 3) and
  * persistent workgroups per CU of the batched launch (1..64; 0 = one
  * workgroup per tile).  Returns 0, or -1 for an unsupported value (nothing
  * changed). */
@@ -181,7 +183,8 @@ int dcx_batch_set_tiling(dcx_batch *batch, int z_chunk, int prefetch_planes,
                          int blocks_per_cu);
 
 /* enabled != 0: the next dcx_batch_run picks the tiling itself, timing
- * candidates on its own first steps (z chunk 32..512, then prefetch depth,
+ * candidates on its own first steps (z chunk 32..512, then prefetch depth
+ * 1 or 3 against 2,
  * then workgroups per CU 1, 2, 4 or one per tile; two steps each).  Every
  * tiling computes the same values, so the run's results do not depend on
  * it.  The choice stays set for later runs (dcx_batch_get_tiling). */

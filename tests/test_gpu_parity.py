@@ -98,6 +98,7 @@ def test_forty_step_loop(H, O, kernel, mode):
 
 @pytest.mark.parametrize("tiling", [(32, 16, 64, 1), (32, 16, 5, 1),
                                     (32, 16, 64, 2), (32, 16, 3, 2),
+                                    (32, 16, 64, 3), (32, 16, 2, 3),
                                     (32, 32, 7, 1), (64, 16, 64, 2)])
 def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
     """Odd sizes (tile overhang in x, y and z) and sub-boxes of the grid."""
@@ -124,6 +125,24 @@ def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
             assert np.array_equal(bits(got[~inside]), bits(old[~inside]))
             assert rel_l2(got[inside] - old[inside],
                           want[inside] - old[inside]) < FAST_TOL
+
+
+def test_prefetch_depths_are_bitwise_equal(H, VM):
+    """The prefetch depth changes only when loads are issued: 1, 2 and 3
+    give the same bits over a fused 30-step loop with a source."""
+    w = VM.random_small((36, 40, 52), seed=12, steps=30)
+    amp = w.source_table()
+    out = []
+    for depth in (1, 2, 3):
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        d.set_tiling(32, 16, 16)
+        assert d.set_prefetch_depth(depth) == 2   # the default
+        d.run(30, w.src_index, amp, graph=True)
+        out.append(d.fields())
+        d.close()
+    for k in (1, 2):
+        for a, b in zip(out[0], out[k]):
+            assert np.array_equal(bits(a), bits(b)), k
 
 
 def test_halo_extract_insert_bitwise(H, O, VM):
