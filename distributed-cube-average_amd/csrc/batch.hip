@@ -53,7 +53,10 @@ dcx_batch *dcx_batch_create(dc_device_data **shots, int n) {
     if (st->fstore != s0->fstore || st->cstore != s0->cstore)
       DCX_FAIL("batch shots must share one storage form");
     if (st->halo_nuser) DCX_FAIL("batch shots cannot carry a halo loopback");
+    if (st->batch) DCX_FAIL("shot %d already belongs to a batch", k);
+    if (st->layer_n) DCX_FAIL("batch shots cannot carry a learned layer");
     adopt_stream(st, b.stream);
+    st->batch = bb;
     b.shots.push_back(st);
   }
   return bb;
@@ -67,6 +70,7 @@ void dcx_batch_free(dcx_batch *bb) {
   for (State *st : b.shots) {
     DCX_CHECK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
     st->own_stream = true;
+    st->batch = nullptr;
   }
   DCX_CHECK(hipEventDestroy(b.ev[0]));
   DCX_CHECK(hipEventDestroy(b.ev[1]));

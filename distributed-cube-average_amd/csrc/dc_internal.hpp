@@ -170,7 +170,11 @@ struct State {
   // dcx_learned_sample_step.
   int sample_w, sample_n;
   uint32_t *sample_cells;
+  // The batch the struct belongs to (dcx_batch_create), if any.
+  const void *batch;
 };
+// Count of structs a network is attached to (it cannot be freed before).
+void mlp_attach(const void *mlp, int delta);
 
 constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"
 constexpr int64_t kRowAlign = 32;  // floats per 128-byte cache line

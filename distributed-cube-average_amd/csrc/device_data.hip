@@ -348,6 +348,7 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
 void dc_device_data_free(dc_device_data *data) {
   if (data == nullptr) return;
   State *st = state_of(data);
+  if (st->batch) DCX_FAIL("free the batch (dcx_batch_free) before its shots");
   DCX_CHECK(hipSetDevice(st->device));
   DCX_CHECK(hipStreamSynchronize(st->stream));
   for (int k = 0; k < 4; ++k) DCX_CHECK(hipFree(st->field_alloc[k]));

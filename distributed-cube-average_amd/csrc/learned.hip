@@ -64,6 +64,7 @@ enum { kOutFree = 0, kOutDamping = 1 };
 
 struct Mlp {
   int device;
+  int attached;  // structs using the network (dcx_set_learned_layer)
   int out_mode;
   float sigma_max;
   hipStream_t stream;
@@ -908,7 +909,12 @@ void layer_apply(State *st) {
   DCX_CHECK(hipGetLastError());
 }
 
+void mlp_attach(const void *mlp, int delta) {
+  if (mlp) const_cast<Mlp *>(static_cast<const Mlp *>(mlp))->attached += delta;
+}
+
 void layer_free(State *st) {
+  mlp_attach(st->layer_mlp, -1);
   if (st->layer_cells) DCX_CHECK(hipFree(st->layer_cells));
   if (st->layer_delta) DCX_CHECK(hipFree(st->layer_delta));
   if (st->sample_cells) DCX_CHECK(hipFree(st->sample_cells));
@@ -966,6 +972,8 @@ int dcx_mlp_set_output(dcx_mlp *p, int mode, float sigma_max) {
 void dcx_mlp_free(dcx_mlp *p) {
   if (p == nullptr) return;
   Mlp *m = mlp_of(p);
+  if (m->attached)
+    DCX_FAIL("network still attached to %d struct(s): detach it first", m->attached);
   DCX_CHECK(hipStreamSynchronize(m->stream));
   DCX_CHECK(hipFree(m->w32));
   DCX_CHECK(hipFree(m->loss));
@@ -1144,8 +1152,10 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *p, int width) {
   st->layer_cells = nullptr;
   st->layer_delta = nullptr;
   st->layer_n = 0;
+  mlp_attach(st->layer_mlp, -1);
   st->layer_mlp = nullptr;
   if (p == nullptr || width == 0) return 0;
+  if (st->batch) DCX_FAIL("a batch shot cannot carry a learned layer");
   if (st->fstore != DCX_STORE_F32 || st->cstore != DCX_COEF_F32)
     DCX_FAIL("the learned layer runs on FP32 storage");
   check_layer_geom(st, width);
@@ -1160,6 +1170,7 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *p, int width) {
   st->layer_n = (int)cells.size();
   st->layer_w = width;
   st->layer_mlp = m;
+  mlp_attach(m, +1);
   return st->layer_n;
 }
 

@@ -13,6 +13,7 @@ CPU fallback, and loading fails loudly if the library has not been built.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import os
 from typing import Sequence
 
@@ -393,10 +394,15 @@ class DeviceData:
 
     def close(self):
         if getattr(self, "ptr", None):
-            load().dc_device_data_free(self.ptr)
+            load().dc_device_data_free(self.ptr)   # also detaches its layer
             self.ptr = None
+            self._layer = None
 
     def __del__(self):
+        # At interpreter exit objects go in no particular order; the process
+        # exit releases device memory, so nothing is freed then.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -455,6 +461,10 @@ class Batch:
             self.ptr = None
 
     def __del__(self):
+        # At interpreter exit objects go in no particular order; the process
+        # exit releases device memory, so nothing is freed then.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -513,6 +523,10 @@ class Mlp:
             self.ptr = None
 
     def __del__(self):
+        # At interpreter exit objects go in no particular order; the process
+        # exit releases device memory, so nothing is freed then.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -552,6 +566,10 @@ class Samples:
             self.ptr = None
 
     def __del__(self):
+        # At interpreter exit objects go in no particular order; the process
+        # exit releases device memory, so nothing is freed then.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -560,9 +578,12 @@ class Samples:
 
 def set_learned_layer(d: DeviceData, mlp, width: int) -> int:
     """Attach (or, with mlp None, detach) the learned layer; returns the
-    number of layer cells."""
-    return load().dcx_set_learned_layer(d.ptr, None if mlp is None else mlp.ptr,
-                                        width)
+    number of layer cells.  The struct keeps the network alive while
+    attached."""
+    n = load().dcx_set_learned_layer(d.ptr, None if mlp is None else mlp.ptr,
+                                     width)
+    d._layer = mlp if (mlp is not None and width) else None
+    return n
 
 
 def learned_sample_step(padded: DeviceData, small: DeviceData, origin, width,

@@ -118,3 +118,55 @@ def test_batch_rejects_mixed_forms(H, shots):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "storage form" in r.stderr
+
+
+def test_full_size_survey_shots(H, VM):
+    """Six shots of the benchmark survey at their real apertures (up to
+    320x320x256 compute cells): batched with the autotuner and a graph, each
+    bit-identical to the shot run alone."""
+    S = VM.layered_survey(seed=0, n_shots=6, steps=40)
+    steps = 24
+    ws = [S.shot(k) for k in range(len(S))]
+    say("survey shots", [w.sizes for w in ws])
+    ds = []
+    for w in ws:
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        d.release_host()
+        ds.append(d)
+    amp = np.stack([w.source_table()[:steps] for w in ws])
+    b = H.Batch(ds)
+    b.set_autotune(True)
+    b.run(steps, [w.src_index for w in ws], amp, graph=True)
+    say("  tiling used", b.tiling())
+    b.close()
+    for k, (w, d) in enumerate(zip(ws, ds)):
+        got = d.fields()
+        d.close()
+        a = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        a.run(steps, w.src_index, amp[k], graph=True)
+        want = a.fields()
+        a.close()
+        for name, g, e in zip(("pp", "pc", "qp", "qc"), got, want):
+            assert np.array_equal(bits(g), bits(e)), (k, name)
+
+
+def test_freeing_a_shot_inside_a_batch_fails_loudly(H):
+    """A shot freed while its batch is open would leave the batch with a
+    dangling struct: the library refuses (message and exit), and a network
+    still attached to a struct cannot be freed either."""
+    import os
+    import subprocess
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prelude = ("import importlib, numpy as np\n"
+               "H = importlib.import_module('distributed-cube-average_amd').dc_hip\n"
+               "c = [np.ones((24, 24, 24), np.float32) for _ in range(10)]\n")
+    cases = [
+        ("a = H.DeviceData(c); b = H.DeviceData(c); B = H.Batch([a, b])\n"
+         "a.close()\n", "free the batch"),
+        ("a = H.DeviceData(c); m = H.Mlp(np.zeros(8450, np.float32))\n"
+         "H.set_learned_layer(a, m, 3)\nm.close()\n", "still attached"),
+    ]
+    for code, msg in cases:
+        r = subprocess.run([sys.executable, "-c", prelude + code], cwd=repo,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode != 0 and msg in r.stderr, (r.returncode, r.stderr[-300:])

@@ -217,3 +217,30 @@ def test_smooth_random_200_steps(H, O, VM):
         f"reference FP32 p={rel_l2(ref[1], f64[1]):.3e} receivers="
         f"{rel_l2(ref[1].reshape(-1)[rec], f64[1].reshape(-1)[rec]):.3e}")
     assert max(e) < FAST_TOL and tr < FAST_TOL
+
+
+def test_full_size_octant_against_oracle(H, O, VM):
+    """The 520^3 octant workload itself (the benchmark's rank): 3 steps with
+    the source, exact kernel bit-identical to the oracle (the reference's
+    arithmetic) on the host, fast kernel within the 1e-5 bar."""
+    w = VM.layered_octant(seed=0, local=512)
+    steps = 3
+    amp = H.ricker_table(w.dt, steps, w.fcut)
+    say("octant 520^3: oracle", steps, "steps")
+    zeros = [np.zeros(w.shape, np.float32) for _ in range(4)]
+    ref = O.run_f32(zeros, w.coef, w.dx, w.dy, w.dz, w.dt, steps, w.src_index, amp)
+    del zeros
+    for kernel in (H.KERNEL_EXACT, H.KERNEL_FAST):
+        say("octant 520^3: kernel", kernel)
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt, kernel=kernel)
+        d.release_host()
+        d.run(steps, w.src_index, amp, graph=True)
+        pc, qc = d.get_results()
+        d.close()
+        if kernel == H.KERNEL_EXACT:
+            assert np.array_equal(bits(pc), bits(ref[1]))
+            assert np.array_equal(bits(qc), bits(ref[3]))
+        else:
+            e = rel_l2(pc, ref[1]), rel_l2(qc, ref[3])
+            say(f"octant fast vs oracle: p={e[0]:.3e} q={e[1]:.3e}")
+            assert max(e) < FAST_TOL
