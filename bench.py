@@ -295,6 +295,28 @@ def batch_study(H, VM, args):
     return out
 
 
+def learned_study(H, args):
+    """The learned absorbing layer at 384^3 (6-cell layer, shipped weights):
+    reflection ratios against a 640^3 padded judge for a source 40 cells
+    below one face over 1000 steps, and the device time per step of the thin
+    domain with and without the layer."""
+    LB = importlib.import_module("distributed-cube-average_amd.learned_boundary")
+    params, meta = LB.load_weights()
+    mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
+    case = LB.make_case(7001, inner=384, width=int(meta["width"]), margin=128,
+                        src_depth=40)
+    r = LB.reflection(mlp, case, 1000)
+    mlp.close()
+    del case
+    plain, learned, ncell = LB.cost(384, int(meta["width"]))
+    out = {"reflection": {k: float(f"{v:.4g}") for k, v in r.items()},
+           "ms_per_step_plain": round(plain, 4),
+           "ms_per_step_learned": round(learned, 4), "layer_cells": ncell,
+           "width": int(meta["width"]), "judge": "640^3 padded domain"}
+    log(f"[bench] learned layer: {out}")
+    return out
+
+
 def pmc_child(args):
     """Run under rocprofv3 --pmc: one calibration copy, then a few steps."""
     pkg = importlib.import_module("distributed-cube-average_amd")
@@ -417,6 +439,8 @@ def main():
     ap.add_argument("--batch-shots", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=800)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--learned", action="store_true",
+                    help="add the 384^3 learned absorbing layer study")
     ap.add_argument("--precision-study", type=int, nargs="*", default=None,
                     help="step counts of the storage-form precision study")
     ap.add_argument("--traffic", action="store_true",
@@ -480,6 +504,9 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_study(H, VM, args)
+    learned = None
+    if args.learned:
+        learned = learned_study(H, args)
     precision = None
     if args.precision_study:
         precision = precision_study(H, VM, args.precision_study)
@@ -521,6 +548,7 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms,
         "batch": batch,
+        "learned_layer": learned,
         "precision": precision,
     }
     print(json.dumps(line), flush=True)

@@ -24,7 +24,7 @@
 // Training reduces over samples, which needs the transposed tiles; they go
 // through LDS.
 #include <cstdint>
-#include <random>
+#include <cmath>
 #include <vector>
 
 #include "dc_internal.hpp"

@@ -331,9 +331,9 @@ def main(argv=None):
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--mode", choices=["free", "damping"], default="damping")
     ap.add_argument("--sigma-max", type=float, default=0.5)
-    ap.add_argument("--rounds", type=int, default=3,
-                    help="closed-loop collection rounds after the first")
-    ap.add_argument("--round-iters", type=int, default=3000)
+    ap.add_argument("--loop-passes", type=int, default=0,
+                    help="closed-loop collection passes after the first")
+    ap.add_argument("--loop-iters", type=int, default=3000)
     ap.add_argument("--eval-margin", type=int, default=96)
     ap.add_argument("--taper", type=int, default=1)
     ap.add_argument("--src-depth", type=int, default=None,
@@ -363,28 +363,28 @@ def main(argv=None):
         return
     if args.cmd == "train":
         per_model = ((args.steps - 60 + 3) // 4) * args.cells
-        samples = H.Samples(per_model * args.models * (1 + args.rounds) + 1)
+        samples = H.Samples(per_model * args.models * (1 + args.loop_passes) + 1)
         seeds = list(range(args.models))
         collect(samples, seeds, steps=args.steps, cells_per_step=args.cells,
                 taper=bool(args.taper), **kw)
         mlp, losses = train(samples, iters=args.iters, batch=args.batch,
                             lr=args.lr, mode=mode, sigma_max=sig)
         scores = evaluate(mlp, "teacher-forced")
-        for r in range(args.rounds):
+        for r in range(args.loop_passes):
             # Closed loop: the small domain runs with the current network.
             collect(samples, [1000 * (r + 1) + k for k in seeds],
                     steps=args.steps, cells_per_step=args.cells, mlp=mlp,
                     rollout=True, taper=bool(args.taper), **kw)
-            mlp, losses = train(samples, iters=args.round_iters,
+            mlp, losses = train(samples, iters=args.loop_iters,
                                 batch=args.batch, lr=args.lr * 0.5,
                                 seed=r + 1, params=mlp.params(), mode=mode,
                                 sigma_max=sig)
-            scores = evaluate(mlp, f"round {r + 1}")
+            scores = evaluate(mlp, f"closed-loop pass {r + 1}")
         params = mlp.params()
         save_weights(args.out, params, dict(
             width=args.width, inner=args.inner, margin=args.margin,
             models=args.models, steps=args.steps, iters=args.iters,
-            rounds=args.rounds, round_iters=args.round_iters,
+            loop_passes=args.loop_passes, loop_iters=args.loop_iters,
             batch=args.batch, lr=args.lr, samples=samples.count(),
             mode=mode, sigma_max=sig, eval_margin=args.eval_margin,
             taper=args.taper,
