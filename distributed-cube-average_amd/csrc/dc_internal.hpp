@@ -161,11 +161,11 @@ struct State {
   float c_lo[NCOEF], c_st[NCOEF];
   int halo_esize;       // element size the loopback tables were built for
   // Learned absorbing layer (dcx_set_learned_layer): the network, the layer
-  // width, its cells (packed compute coordinates) and their corrections.
+  // width, its cell count, its tiles and their corrections.
   const void *layer_mlp;
-  int layer_w, layer_n;
-  uint32_t *layer_cells;
-  float *layer_delta;
+  int layer_w, layer_n, layer_ntiles;
+  void *layer_tiles;   // 16 x 16 face-plane tiles of the layer
+  float *layer_delta;  // corrections, per tile cell
   // Layer cells used when the struct is the small domain of
   // dcx_learned_sample_step.
   int sample_w, sample_n;

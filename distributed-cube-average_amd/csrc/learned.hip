@@ -262,69 +262,88 @@ __device__ inline int slot_desc(int s) {
   return k << 1;
 }
 
-// The loads of one group of 16 cells for one lane, kept apart from their use
-// so that the next group's loads fly while the current group runs on MFMA.
-// Offsets are 32-bit (a layer domain's arrays hold fewer than 2^31 floats)
-// and every K step reads one pair of arrays, so each load is a uniform base
-// plus a per-lane offset.
-struct GroupLoads {
-  float a[4][4], b[4][4];
-  float vx, vz;
-  int own, depth;
+// ------------------------------------------------ layer inference by slabs
+//
+// The layer is cut into tiles of 16 x 16 cells of one face plane, all
+// `width` cells deep.  A workgroup stages the time differences D = f^n -
+// f^{n-1} of its tile's box, one cell beyond it on every side, in LDS once,
+// then gathers every cell's patch from there.  A cell belongs to the face
+// cell_frame picks for it; the tiles of other faces skip it.
+
+constexpr int kTile = 16;                         // cells per tile side
+// Staged box: x extent padded to an odd row stride (no LDS bank aliasing of
+// the rows lanes read together); at most (8 + 2 + 1) x 18 x 18 cells.
+constexpr int kBoxMax = (kMaxWidth + 3) * (kTile + 2) * (kTile + 2);
+
+struct SlabTile {
+  int face, t1, t2, pad;  // face = 2 * axis + side; tangent origins
 };
 
-__device__ inline void group_load(GroupLoads &L, const int desc[4][4],
-                                  bool valid, uint32_t code, const LayerGeom &g,
-                                  const float *pc, const float *pp,
-                                  const float *qc, const float *qp) {
-  int own = 0, sn = 0, s1 = 0, s2 = 0, depth = 0;
-  if (valid) {
-    const CellFrame f = cell_frame(code, g);
-    own = (int)f.own;
-    sn = (int)f.sn;
-    s1 = (int)f.s1;
-    s2 = (int)f.s2;
-    depth = f.depth;
+// The tile's cell e (t1 fastest, then t2, then depth) in compute
+// coordinates, and whether it is a layer cell of this face.
+__device__ inline bool slab_cell(const SlabTile &T, int e, const LayerGeom &g,
+                                 int c[3]) {
+  const int axis = T.face >> 1, side = T.face & 1;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const int d = e / (kTile * kTile);
+  c[a1] = T.t1 + e % kTile;
+  c[a2] = T.t2 + (e / kTile) % kTile;
+  c[axis] = side ? n[axis] - 1 - d : d;
+  if (d >= g.width || c[a1] >= n[a1] || c[a2] >= n[a2]) return false;
+  // Nearest face by cell_frame's rule (lowest axis, low side first).
+  int best = 1 << 30, bf = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] < best) { best = c[a]; bf = 2 * a; }
+    if (n[a] - 1 - c[a] < best) { best = n[a] - 1 - c[a]; bf = 2 * a + 1; }
   }
-  L.own = own;
-  L.depth = depth;
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
-    const float *cur = ks < 2 ? pc : qc;
-    const float *prev = ks < 2 ? pp : qp;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int d = desc[ks][r];
-      const int o = (d & 1) ? own + (((d >> 1) & 3) - 1) * sn +
-                                  (((d >> 3) & 3) - 1) * s1 + (((d >> 5) & 3) - 1) * s2
-                            : own;
-      L.a[ks][r] = cur[(uint32_t)o];
-      L.b[ks][r] = prev[(uint32_t)o];
-    }
-  }
-  L.vx = g.v2px[(uint32_t)own];
-  L.vz = g.v2pz[(uint32_t)own];
+  return bf == T.face;
+}
+
+// Box of a tile in compute coordinates: origin and extent per axis.
+__device__ inline void slab_box(const SlabTile &T, const LayerGeom &g, int lo[3],
+                                int len[3]) {
+  const int axis = T.face >> 1, side = T.face & 1;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  lo[axis] = side ? n[axis] - 1 - g.width : -1;
+  len[axis] = g.width + 2;
+  lo[a1] = T.t1 - 1;
+  len[a1] = kTile + 2;
+  lo[a2] = T.t2 - 1;
+  len[a2] = kTile + 2;
+}
+
+__device__ inline int64_t dev_offset(const LayerGeom &g, int x, int y, int z) {
+  return (int64_t)(x + STENCIL) + (int64_t)(y + STENCIL) * g.ys +
+         (int64_t)(z + STENCIL) * g.zs;
 }
 
 // Corrections of the layer cells from the current fields (before the step).
-// Weights and biases are staged in LDS (the A operands are read per MFMA) to
-// leave registers for the loads in flight; W1 is the slot-ordered copy.
+// Weights and biases are staged in LDS too (the A operands are read per
+// MFMA); W1 is the slot-ordered copy.
 __global__ void __launch_bounds__(256)
     k_layer_infer(const float *pc, const float *pp, const float *qc,
-                  const float *qp, LayerGeom g, const uint32_t *__restrict__ cells,
-                  int ncell, const _Float16 *__restrict__ wh,
-                  const float *__restrict__ w32, float2 *__restrict__ delta,
-                  int mode, float sigma_max) {
-  constexpr int lW1 = 0, lW2 = 1024, lW3 = 2048;  // h4 offsets in sW
-  __shared__ h4 sW[2048 + 256];
+                  const float *qp, LayerGeom g, const SlabTile *__restrict__ tiles,
+                  const _Float16 *__restrict__ wh, const float *__restrict__ w32,
+                  float2 *__restrict__ delta, int mode, float sigma_max) {
+  // A operands in LDS with rows padded from 16 to 18 h4 (72 halves): lane
+  // (col, kb) reads 8 bytes at row col, so the 64 lanes of a read cover
+  // every bank exactly twice instead of 8 banks eight times.
+  constexpr int kRow = 18;
+  constexpr int lW1 = 0, lW2 = 64 * kRow, lW3 = 128 * kRow;  // h4 offsets
+  __shared__ h4 sW[144 * kRow];
   __shared__ float sB[kHid + kHid + 16];
   __shared__ int sDesc[kIn];
+  __shared__ float sD[2][kBoxMax];
+  const SlabTile T = tiles[blockIdx.x];
   for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
-    sW[lW1 + t] = reinterpret_cast<const h4 *>(wh + hW1p)[t];
-    sW[lW2 + t] = reinterpret_cast<const h4 *>(wh + hW2)[t];
+    const int row = t >> 4, q = t & 15;
+    sW[lW1 + row * kRow + q] = reinterpret_cast<const h4 *>(wh + hW1p)[t];
+    sW[lW2 + row * kRow + q] = reinterpret_cast<const h4 *>(wh + hW2)[t];
   }
   for (int t = threadIdx.x; t < 256; t += blockDim.x)
-    sW[lW3 + t] = reinterpret_cast<const h4 *>(wh + hW3)[t];
+    sW[lW3 + (t >> 4) * kRow + (t & 15)] = reinterpret_cast<const h4 *>(wh + hW3)[t];
   for (int t = threadIdx.x; t < kHid; t += blockDim.x) {
     sB[t] = w32[oB1 + t];
     sB[kHid + t] = w32[oB2 + t];
@@ -332,65 +351,87 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x < 16)
     sB[2 * kHid + threadIdx.x] = threadIdx.x < kOut ? w32[oB3 + threadIdx.x] : 0.0f;
   if (threadIdx.x < kIn) sDesc[threadIdx.x] = slot_desc(threadIdx.x);
+  // The box, x fastest; coordinates outside the arrays are clamped (those
+  // box cells are beyond the tile's last cell and never read).
+  int lo[3], len[3];
+  slab_box(T, g, lo, len);
+  const int nbox = len[0] * len[1] * len[2];
+  const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};  // LDS strides
+  const int n[3] = {g.nx, g.ny, g.nz};
+  for (int e = threadIdx.x; e < nbox; e += blockDim.x) {
+    const int bx = e % len[0], by = (e / len[0]) % len[1], bz = e / (len[0] * len[1]);
+    int c[3] = {lo[0] + bx, lo[1] + by, lo[2] + bz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) c[a] = min(max(c[a], -STENCIL), n[a] + STENCIL - 1);
+    const uint32_t o = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
+    const int l = bx + by * ls[1] + bz * ls[2];
+    sD[0][l] = pc[o] - pp[o];
+    sD[1][l] = qc[o] - qp[o];
+  }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int nwave = gridDim.x * (blockDim.x / 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, kb = lane >> 4;
   int desc[4][4];
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
     for (int r = 0; r < 4; ++r) desc[ks][r] = sDesc[16 * ks + 4 * kb + r];
-  const int stride = nwave * 16;
-  int g0 = wave * 16;
-  GroupLoads ld;
-  {
-    const int c = g0 + col;
-    group_load(ld, desc, c < ncell, c < ncell ? cells[c] : 0u, g, pc, pp, qc, qp);
-  }
-  for (; g0 < ncell; g0 += stride) {
-    const int c = g0 + col;
-    const bool valid = c < ncell;
+  // LDS strides of the tile's frame: outward normal, tangent 1, tangent 2.
+  const int axis = T.face >> 1, side = T.face & 1;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
+  const int ncell = kTile * kTile * g.width;
+  // Every wave takes the same number of groups (ncell is a multiple of 64).
+  for (int g0 = wave * 16; g0 < ncell; g0 += 64) {
     // An offset the compiler cannot see is zero: it keeps the LDS operand
     // reads inside the loop instead of holding all 36 in registers.
     int z = 0;
     asm volatile("" : "+s"(z));
-    // Consume this group's loads (the same arithmetic as raw_feature and
-    // norm_feature), then start the next group's.
+    const int e = g0 + col;
+    int c[3];
+    const bool valid = slab_cell(T, e, g, c);
+    const int l = (c[0] - lo[0]) + (c[1] - lo[1]) * ls[1] + (c[2] - lo[2]) * ls[2];
+    const int d = e / (kTile * kTile);
     float v[4][4];
     float m = 0.0f;
+    float vx = 0.0f, vz = 0.0f;
+    if (valid) {
+      const uint32_t og = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
+      vx = g.v2px[og];
+      vz = g.v2pz[og];
+    }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int d = desc[ks][r];
-        float x;
-        if (d & 1) {
-          x = ld.a[ks][r] - ld.b[ks][r];
-          m = fmaxf(m, fabsf(x));
-        } else {
-          const int k = d >> 1;
-          x = k < kCourant ? ((k - kPatch) == ld.depth ? 1.0f : 0.0f)
-                           : sqrtf(k == kCourant ? ld.vx : ld.vz) * g.cdt;
+        const int dd = desc[ks][r];
+        float x = 0.0f;
+        if (valid) {
+          if (dd & 1) {
+            const int o = l + (((dd >> 1) & 3) - 1) * sn + (((dd >> 3) & 3) - 1) * s1 +
+                          (((dd >> 5) & 3) - 1) * s2;
+            x = sD[ks < 2 ? 0 : 1][o];
+            m = fmaxf(m, fabsf(x));
+          } else {
+            const int k = dd >> 1;
+            x = k < kCourant ? ((k - kPatch) == d ? 1.0f : 0.0f)
+                             : sqrtf(k == kCourant ? vx : vz) * g.cdt;
+          }
         }
-        v[ks][r] = valid ? x : 0.0f;
+        v[ks][r] = x;
       }
-    const int own = ld.own;
-    {
-      const int cn = g0 + stride + col;
-      const bool vn = cn < ncell;
-      group_load(ld, desc, vn, vn ? cells[cn] : 0u, g, pc, pp, qc, qp);
-    }
     m = fmaxf(m, __shfl_xor(m, 16));
     m = fmaxf(m, __shfl_xor(m, 32));
+    // One reciprocal per cell instead of 54 divisions; the inputs are
+    // rounded to half right after, which hides the last-bit difference.
+    const float rm = m > 0.0f ? 1.0f / m : 0.0f;
     h4 X[4];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float x = v[ks][r];
-        X[ks][r] = (_Float16)((desc[ks][r] & 1) ? (m > 0.0f ? x / m : 0.0f) : x);
+        X[ks][r] = (_Float16)((desc[ks][r] & 1) ? x * rm : x);
       }
     h4 H1[4], H2[4];
 #pragma unroll
@@ -401,7 +442,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks)
         acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-            sW[z + lW1 + ((16 * ot + col) * 64 + 16 * ks + 4 * kb) / 4], X[ks], acc, 0, 0, 0);
+            sW[z + lW1 + (16 * ot + col) * kRow + 4 * ks + kb], X[ks], acc, 0, 0, 0);
       H1[ot] = relu_h(acc);
     }
 #pragma unroll
@@ -412,7 +453,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks)
         acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-            sW[z + lW2 + ((16 * ot + col) * 64 + 16 * ks + 4 * kb) / 4], H1[ks], acc, 0, 0, 0);
+            sW[z + lW2 + (16 * ot + col) * kRow + 4 * ks + kb], H1[ks], acc, 0, 0, 0);
       H2[ot] = relu_h(acc);
     }
     f4 y;
@@ -421,34 +462,40 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       y = __builtin_amdgcn_mfma_f32_16x16x16f16(
-          sW[z + lW3 + (col * 64 + 16 * ks + 4 * kb) / 4], H2[ks], y, 0, 0, 0);
+          sW[z + lW3 + col * kRow + 4 * ks + kb], H2[ks], y, 0, 0, 0);
     if (lane < 16 && valid) {
+      float2 out;
       if (mode == kOutDamping) {
-        // The cell's own time differences (inputs 13 and 40 are held by
-        // other lanes): read again, from cache.
         const float sp = sigma_max / (1.0f + expf(-y[0]));
         const float sq = sigma_max / (1.0f + expf(-y[1]));
-        delta[c] = make_float2(-sp * (pc[own] - pp[own]), -sq * (qc[own] - qp[own]));
+        out = make_float2(-sp * sD[0][l], -sq * sD[1][l]);
       } else {
-        delta[c] = make_float2(m * y[0], m * y[1]);
+        out = make_float2(m * y[0], m * y[1]);
       }
+      delta[(size_t)blockIdx.x * ncell + e] = out;
     }
   }
 }
 
 // Add the corrections to the new fields the stencil wrote.
-__global__ void k_layer_apply(float *pn, float *qn, LayerGeom g,
-                              const uint32_t *__restrict__ cells, int ncell,
-                              const float2 *__restrict__ delta) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  const uint32_t code = cells[c];
-  const int64_t o = (int64_t)(code & 1023u) + STENCIL +
-                    ((int64_t)((code >> 10) & 1023u) + STENCIL) * g.ys +
-                    ((int64_t)((code >> 20) & 1023u) + STENCIL) * g.zs;
-  const float2 d = delta[c];
-  pn[o] += d.x;
-  qn[o] += d.y;
+__global__ void __launch_bounds__(256)
+    k_layer_apply(float *pn, float *qn, LayerGeom g,
+                  const SlabTile *__restrict__ tiles,
+                  const float2 *__restrict__ delta) {
+  const SlabTile T = tiles[blockIdx.x];
+  const int w = g.width;
+  const int ncell = kTile * kTile * w;
+  const bool xface = (T.face >> 1) == 0;
+  for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+    // x faces: depth (= x) fastest, so that lanes write along memory rows.
+    const int e = xface ? (i % w) * kTile * kTile + i / w : i;
+    int c[3];
+    if (!slab_cell(T, e, g, c)) continue;
+    const int64_t o = dev_offset(g, c[0], c[1], c[2]);
+    const float2 d = delta[(size_t)blockIdx.x * ncell + e];
+    pn[o] += d.x;
+    qn[o] += d.y;
+  }
 }
 
 // MLP output y (unscaled) of stored samples (tests).
@@ -893,9 +940,9 @@ void layer_infer(State *st, const StepArgs &a) {
     DCX_FAIL("the learned layer runs on FP32 storage");
   const Mlp *m = reinterpret_cast<const Mlp *>(st->layer_mlp);
   const LayerGeom g = geom_of(st, st->layer_w, a.dt * sqrtf(a.dxx));
-  hipLaunchKernelGGL(k_layer_infer, dim3(mlp_grid(st->layer_n)), dim3(256), 0,
+  hipLaunchKernelGGL(k_layer_infer, dim3(st->layer_ntiles), dim3(256), 0,
                      st->stream, st->abi.pc, st->abi.pp, st->abi.qc, st->abi.qp,
-                     g, st->layer_cells, st->layer_n, m->wh, m->w32,
+                     g, (const SlabTile *)st->layer_tiles, m->wh, m->w32,
                      (float2 *)st->layer_delta, m->out_mode, m->sigma_max);
   DCX_CHECK(hipGetLastError());
 }
@@ -903,9 +950,10 @@ void layer_infer(State *st, const StepArgs &a) {
 void layer_apply(State *st) {
   if (st->layer_n == 0) return;
   const LayerGeom g = geom_of(st, st->layer_w);
-  hipLaunchKernelGGL(k_layer_apply, dim3((st->layer_n + 255) / 256), dim3(256),
-                     0, st->stream, st->abi.pp, st->abi.qp, g, st->layer_cells,
-                     st->layer_n, (const float2 *)st->layer_delta);
+  hipLaunchKernelGGL(k_layer_apply, dim3(st->layer_ntiles), dim3(256), 0,
+                     st->stream, st->abi.pp, st->abi.qp, g,
+                     (const SlabTile *)st->layer_tiles,
+                     (const float2 *)st->layer_delta);
   DCX_CHECK(hipGetLastError());
 }
 
@@ -915,10 +963,10 @@ void mlp_attach(const void *mlp, int delta) {
 
 void layer_free(State *st) {
   mlp_attach(st->layer_mlp, -1);
-  if (st->layer_cells) DCX_CHECK(hipFree(st->layer_cells));
+  if (st->layer_tiles) DCX_CHECK(hipFree(st->layer_tiles));
   if (st->layer_delta) DCX_CHECK(hipFree(st->layer_delta));
   if (st->sample_cells) DCX_CHECK(hipFree(st->sample_cells));
-  st->layer_cells = nullptr;
+  st->layer_tiles = nullptr;
   st->layer_delta = nullptr;
   st->sample_cells = nullptr;
   st->layer_n = 0;
@@ -1147,11 +1195,12 @@ double dcx_mlp_train(dcx_mlp *p, const dcx_samples *sp, int n_iter, int batch,
 int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *p, int width) {
   State *st = state_of(data);
   DCX_CHECK(hipStreamSynchronize(st->stream));
-  if (st->layer_cells) DCX_CHECK(hipFree(st->layer_cells));
+  if (st->layer_tiles) DCX_CHECK(hipFree(st->layer_tiles));
   if (st->layer_delta) DCX_CHECK(hipFree(st->layer_delta));
-  st->layer_cells = nullptr;
+  st->layer_tiles = nullptr;
   st->layer_delta = nullptr;
   st->layer_n = 0;
+  st->layer_ntiles = 0;
   mlp_attach(st->layer_mlp, -1);
   st->layer_mlp = nullptr;
   if (p == nullptr || width == 0) return 0;
@@ -1162,12 +1211,25 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *p, int width) {
   Mlp *m = mlp_of(p);
   if (m->device != st->device) DCX_FAIL("network and struct on different devices");
   DCX_CHECK(hipStreamSynchronize(m->stream));
-  const std::vector<uint32_t> cells = layer_cells(geom_of(st, width));
-  DCX_CHECK(hipMalloc(&st->layer_cells, cells.size() * sizeof(uint32_t)));
-  DCX_CHECK(hipMemcpy(st->layer_cells, cells.data(), cells.size() * sizeof(uint32_t),
+  // Tiles of 16 x 16 cells of each face plane.
+  const int64_t n[3] = {st->sizes[0] - 2 * STENCIL, st->sizes[1] - 2 * STENCIL,
+                        st->sizes[2] - 2 * STENCIL};
+  std::vector<SlabTile> tiles;
+  for (int f = 0; f < 6; ++f) {
+    const int axis = f >> 1;
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    for (int t2 = 0; t2 < n[a2]; t2 += kTile)
+      for (int t1 = 0; t1 < n[a1]; t1 += kTile) tiles.push_back(SlabTile{f, t1, t2, 0});
+  }
+  DCX_CHECK(hipMalloc(&st->layer_tiles, tiles.size() * sizeof(SlabTile)));
+  DCX_CHECK(hipMemcpy(st->layer_tiles, tiles.data(), tiles.size() * sizeof(SlabTile),
                       hipMemcpyHostToDevice));
-  DCX_CHECK(hipMalloc(&st->layer_delta, cells.size() * 2 * sizeof(float)));
-  st->layer_n = (int)cells.size();
+  DCX_CHECK(hipMalloc(&st->layer_delta,
+                      tiles.size() * (size_t)kTile * kTile * width * 2 * sizeof(float)));
+  int64_t inner = 1;
+  for (int a = 0; a < 3; ++a) inner *= n[a] - 2 * width;
+  st->layer_n = (int)(n[0] * n[1] * n[2] - inner);
+  st->layer_ntiles = (int)tiles.size();
   st->layer_w = width;
   st->layer_mlp = m;
   mlp_attach(m, +1);
