@@ -28,7 +28,7 @@ HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-I" + INC, "-I" + CSRC]
 HOST_FLAGS = ["-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c++17",
-              "-Wall", "-I" + INC]
+              "-Wall", "-I" + INC, "-I" + CSRC]
 
 HIP_SOURCES = ["stencil.hip", "halo.hip", "device_data.hip", "program.hip", "batch.hip", "learned.hip",
                "storage.hip"]

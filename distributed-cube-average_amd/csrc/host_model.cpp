@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 
+#include "boundary_draw.hpp"
 #include "dc_hip.h"
 
 extern "C" {
@@ -57,60 +58,47 @@ void dcx_precompute(size_t n, const float *vpz, const float *vsv,
   }
 }
 
-// src/boundary.c:7-104.  The draw order (one rand_r per shell cell of the
-// whole global grid, z then y then x) is what makes every rank agree.
+// src/boundary.c:7-104, cell by cell: each local cell finds its own draw
+// from its index among the global grid's shell cells (boundary_draw.hpp), so
+// the work is O(local cells) and parallel instead of a walk over the global
+// grid; *seed ends where the reference's walk leaves it.
 void dcx_random_boundary(int local_sx, int local_sy, int local_sz,
                          int global_sx, int global_sy, int global_sz,
                          int start_x, int start_y, int start_z, int nx, int ny,
                          int nz, int bord, int absorb, float *vpz, float *vsv,
                          unsigned int *seed) {
-  const int last = bord + absorb - 1;  // last shell index before the interior
-  const int first = last + 1;          // first interior index
-  const float frac = 1.0f / (float)absorb;
-  const float maxP = 3000.0f;
-  const float maxS = maxP * sqrtf(fabsf(0.24f - 0.1f) / 0.75f);
-  const int ex = start_x + local_sx, ey = start_y + local_sy,
-            ez = start_z + local_sz;
-  auto dist = [](int g, int first_, int last_, int n_) {
-    if (g > last_ + n_) return g - last_ - n_;
-    if (g < first_) return first_ - g;
-    return 0;
-  };
-  for (int gz = 0; gz < global_sz; gz++) {
-    for (int gy = 0; gy < global_sy; gy++) {
-      for (int gx = 0; gx < global_sx; gx++) {
-        const bool interior = gz >= first && gz <= last + nz && gy >= first &&
-                              gy <= last + ny && gx >= first && gx <= last + nx;
-        if (interior) continue;
-        const bool shell = gz >= bord && gz <= 2 * last + nz && gy >= bord &&
-                           gy <= 2 * last + ny && gx >= bord &&
-                           gx <= 2 * last + nx;
-        const bool mine = gx >= start_x && gx < ex && gy >= start_y &&
-                          gy < ey && gz >= start_z && gz < ez;
-        if (shell) {
-          const float r = (float)rand_r(seed) / (float)RAND_MAX;
-          if (!mine) continue;
-          const size_t i = (size_t)(gx - start_x) +
-                           (size_t)(gy - start_y) * (size_t)local_sx +
-                           (size_t)(gz - start_z) * (size_t)local_sx * local_sy;
-          int d = dist(gy, first, last, ny);
-          const int dz = dist(gz, first, last, nz);
-          d = d > dz ? d : dz;
-          const int dx = dist(gx, first, last, nx);
-          d = d > dx ? d : dx;
-          const float bd = (float)d * frac;
-          vpz[i] = maxP * (1.0f - bd) + maxP * r * bd;
-          vsv[i] = maxS * (1.0f - bd) + maxS * r * bd;
-        } else if (mine) {
-          const size_t i = (size_t)(gx - start_x) +
-                           (size_t)(gy - start_y) * (size_t)local_sx +
-                           (size_t)(gz - start_z) * (size_t)local_sx * local_sy;
+  (void)global_sx;
+  (void)global_sy;
+  (void)global_sz;
+  const dcx::ShellGeom s{{nx, ny, nz}, bord, absorb};
+  const uint32_t seed0 = *seed;
+  const int64_t plane = (int64_t)local_sx * local_sy;
+  // The shell cells of one row have consecutive draw indices: one jump per
+  // row, then the generator runs on from draw to draw.
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int lz = 0; lz < local_sz; lz++)
+    for (int ly = 0; ly < local_sy; ly++) {
+      uint32_t st = 0;
+      bool have = false;
+      for (int lx = 0; lx < local_sx; lx++) {
+        const int g[3] = {start_x + lx, start_y + ly, start_z + lz};
+        const int kind = dcx::shell_kind(s, g);
+        if (kind == 0) continue;
+        const size_t i = (size_t)(lx + (int64_t)ly * local_sx + lz * plane);
+        if (kind == 2) {
           vpz[i] = 0.0f;
           vsv[i] = 0.0f;
+          continue;
         }
+        if (!have) {
+          st = dcx::lcg_jump(seed0, 3 * dcx::shell_index(s, g));
+          have = true;
+        }
+        const float r = dcx::draw_fraction(dcx::rand_r_draw(st));
+        dcx::shell_velocity(s, g, r, &vpz[i], &vsv[i]);
       }
     }
-  }
+  *seed = dcx::lcg_jump(seed0, 3 * dcx::shell_count(s));
 }
 
 // src/calculate_source.c:5-17; fcut is the reference's FCUT (40).
