@@ -17,9 +17,9 @@ stencil kernel's average launch time comes from HIP events around each
 launch on the library's stream (a second pass), and gives the roofline
 fraction against MI355X's 8 TB/s with 64 algorithmic bytes per cell-update
 (read pc, qc, pp, qp and ten coefficient arrays, write pp, qp).  HBM traffic
-per launch, when --traffic is given, is read from two rocprofv3 --pmc child
-runs (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a copy kernel of
-known size.  The CPU baseline runs the reference's own OpenMP dc_propagate
+per launch is read from two rocprofv3 --pmc child runs (FETCH_SIZE,
+WRITE_SIZE; --no-traffic skips them) scaled by the same counters on a copy
+kernel of known size.  The CPU baseline runs the reference's own OpenMP dc_propagate
 (oracle/_ref, when built) or the oracle's C restatement on the host cores,
 on a bounded sample of the same workload: a few steps of the 520^3 arrays
 from the GPU's field state after the timed steps.
@@ -443,8 +443,9 @@ def main():
                     help="add the 384^3 learned absorbing layer study")
     ap.add_argument("--precision-study", type=int, nargs="*", default=None,
                     help="step counts of the storage-form precision study")
-    ap.add_argument("--traffic", action="store_true",
-                    help="measure HBM bytes per launch with rocprofv3 --pmc")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 --pmc runs that measure HBM bytes "
+                         "per stencil launch (traffic is then null)")
     ap.add_argument("--child-pmc", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -458,7 +459,7 @@ def main():
         return
 
     traffic = None
-    if args.traffic:
+    if not args.no_traffic:
         traffic = measure_traffic(args)
 
     pkg = importlib.import_module("distributed-cube-average_amd")
