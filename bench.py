@@ -16,7 +16,9 @@ synchronisations (wall clock); value = 512^3 * K / seconds / 1e6.  The
 stencil kernel's average launch time comes from HIP events around each
 launch on the library's stream (a second pass), and gives the roofline
 fraction against MI355X's 8 TB/s with 64 algorithmic bytes per cell-update
-(read pc, qc, pp, qp and ten coefficient arrays, write pp, qp).  HBM traffic
+(read pc, qc, pp, qp and ten coefficient arrays, write pp, qp), and against
+a stencil-free pass that streams the same 64 B per cell over the same arrays
+(dcx_stream_probe: what the memory system gives this byte pattern).  HBM traffic
 per launch is read from two rocprofv3 --pmc child runs (FETCH_SIZE,
 WRITE_SIZE; --no-traffic skips them) scaled by the same counters on a copy
 kernel of known size.  The CPU baseline runs the reference's own OpenMP dc_propagate
@@ -501,6 +503,13 @@ def main():
     fields = d.fields()
     copy_gbs = copy_bandwidth()
     log(f"[bench] device-to-device copy of 1 GiB: {copy_gbs:.0f} GB/s (read + write)")
+    # The step's 64 B per cell streamed pointwise with no stencil, over the
+    # same arrays (it overwrites pp and qp, so it runs after the readback).
+    probe_ms = d.stream_probe(20)
+    stream_gbs = alg_bytes / (probe_ms * 1e-3) / 1e9
+    log(f"[bench] stencil-free pass of the same 64 B/cell: {probe_ms:.3f} ms "
+        f"-> {stream_gbs:.0f} GB/s; the stencil reaches "
+        f"{achieved / stream_gbs:.1%} of it")
     d.close()
     batch = None
     if not args.no_batch:
@@ -529,6 +538,9 @@ def main():
     if copy_gbs is not None:
         roof["copy_gbs"] = round(copy_gbs, 1)
         roof["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    roof["stream_ms"] = round(probe_ms, 4)
+    roof["stream_gbs"] = round(stream_gbs, 1)
+    roof["frac_of_stream"] = round(achieved / stream_gbs, 4)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -94,6 +94,8 @@ void launch_fast_batch(const StepArgs *shots, const int64_t *prefix,
 bool fast_tiling_supported(int tile_x, int tile_y);
 void launch_add_source(float *pc, float *qc, int64_t index, float amp,
                        hipStream_t s);
+// Pointwise pass with the step's byte pattern and no stencil (FP32 arrays).
+void launch_stream_probe(const StepArgs &a, hipStream_t s);
 
 // One box copy between a strided array and a packed buffer (elements of
 // esize bytes: 4 for FP32 fields, 2 for 16-bit storage).

@@ -590,6 +590,22 @@ double dcx_last_kernel_ms(const dc_device_data *data) {
   return state_of(data)->last_kernel_ms;
 }
 
+double dcx_stream_probe(dc_device_data *data, int reps) {
+  State *st = state_of(data);
+  if (st->fstore != DCX_STORE_F32 || st->cstore != DCX_COEF_F32)
+    DCX_FAIL("the stream probe reads FP32 fields and coefficients");
+  if (reps < 1) DCX_FAIL("reps must be positive");
+  const StepArgs a = make_step_args(st, full_box(st), 1.0f, 1.0f, 1.0f, 0.0f);
+  launch_stream_probe(a, st->stream);  // untimed first pass
+  DCX_CHECK(hipEventRecord(st->ev[0], st->stream));
+  for (int r = 0; r < reps; ++r) launch_stream_probe(a, st->stream);
+  DCX_CHECK(hipEventRecord(st->ev[1], st->stream));
+  DCX_CHECK(hipEventSynchronize(st->ev[1]));
+  float ms = 0.0f;
+  DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
+  return (double)ms / reps;
+}
+
 }  // extern "C"
 
 // Receiver sampling after each step: out[step * n + r] = p at dev_idx[r]

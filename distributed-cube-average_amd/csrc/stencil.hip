@@ -708,4 +708,33 @@ void launch_add_source(float *pc, float *qc, int64_t index, float amp,
   DCX_CHECK(hipGetLastError());
 }
 
+// The time step's bytes without its stencil: every cell of the box reads
+// pc, qc, pp, qp and the ten coefficients at its own index and writes pp and
+// qp (a pointwise leapfrog with the coefficients summed in), 64 B per cell
+// over the same pitched arrays.  The rate the memory system gives this
+// 16-stream pattern is the ceiling for a step that moves the same bytes.
+// One thread per cell, 256-cell rows along x.
+__global__ void __launch_bounds__(256) k_stream_probe(const StepArgs a) {
+  const int64_t x = a.box.x0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t y = a.box.y0 + blockIdx.y;
+  const int64_t z = a.box.z0 + blockIdx.z;
+  if (x >= a.box.x1) return;
+  const int64_t i = x + y * a.ys + z * a.zs;
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < NCOEF; ++k) s += a.coef.c[k][i];
+  a.pp[i] = 2.0f * a.pc[i] - a.pp[i] + s * a.dt;
+  a.qp[i] = 2.0f * a.qc[i] - a.qp[i] + s * a.dt;
+}
+
+void launch_stream_probe(const StepArgs &a, hipStream_t s) {
+  const int64_t nx = a.box.x1 - a.box.x0, ny = a.box.y1 - a.box.y0,
+                nz = a.box.z1 - a.box.z0;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
+  if (ny > 65535 || nz > 65535) DCX_FAIL("probe box too large");
+  dim3 grid((unsigned)((nx + 255) / 256), (unsigned)ny, (unsigned)nz);
+  hipLaunchKernelGGL(k_stream_probe, grid, dim3(256), 0, s, a);
+  DCX_CHECK(hipGetLastError());
+}
+
 }  // namespace dcx

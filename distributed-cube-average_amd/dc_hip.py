@@ -110,6 +110,7 @@ SIGNATURES = {
                                     C.c_int]),
     "dcx_set_timing_detail": (None, [_DD, C.c_int]),
     "dcx_last_kernel_ms": (C.c_double, [_DD]),
+    "dcx_stream_probe": (C.c_double, [_DD, C.c_int]),
     "dcx_pack_boxes": (None, [_DD, _FP, C.c_int, C.POINTER(C.c_size_t),
                               C.POINTER(C.c_size_t), _FP]),
     "dcx_unpack_boxes": (None, [_DD, _FP, C.c_int, C.POINTER(C.c_size_t),
@@ -361,6 +362,11 @@ class DeviceData:
                (send_boxes, recv_boxes) for i in (0, 1)]
         ptrs = [a.ctypes.data_as(C.POINTER(C.c_size_t)) for a in arr]
         load().dcx_set_halo_loopback(self.ptr, n, *ptrs)
+
+    def stream_probe(self, reps: int = 20) -> float:
+        """ms per pointwise pass with the step's bytes and no stencil
+        (overwrites pp and qp)."""
+        return load().dcx_stream_probe(self.ptr, reps)
 
     def copy_field(self, dst: int, src: int) -> int:
         """Calibration copy of a whole allocation; returns bytes read."""

@@ -127,6 +127,13 @@ double dcx_run_traces(dc_device_data *data, float dx, float dy, float dz,
 void dcx_set_timing_detail(dc_device_data *data, int enabled);
 double dcx_last_kernel_ms(const dc_device_data *data);
 
+/* Average duration (ms) of `reps` pointwise passes over the compute region
+ * that move the time step's bytes without its stencil: each cell reads pc,
+ * qc, pp, qp and the ten coefficients and writes pp and qp (64 B per cell,
+ * FP32 storage only).  Overwrites pp and qp.  The rate of this pass is the
+ * ceiling the memory system sets for a step of the same bytes. */
+double dcx_stream_probe(dc_device_data *data, int reps);
+
 /* ------------------------------------------------- halo boxes on device -- */
 
 /* Pack nbox boxes of the device array `from` (a field of data, laid out with

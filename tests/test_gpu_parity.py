@@ -145,6 +145,28 @@ def test_prefetch_depths_are_bitwise_equal(H, VM):
             assert np.array_equal(bits(a), bits(b)), k
 
 
+def test_stream_probe_touches_the_compute_region_only(H, VM):
+    """The stencil-free probe (one untimed pass + reps timed passes) writes
+    pp <- 2 pc - pp + dt * sum(coef) with dt = 0 at every compute cell and
+    nothing else: three passes leave 2 pc - pp there, the rest bit for bit."""
+    shape = (21, 26, 300)  # (sz, sy, sx): two 256-cell rows along x
+    w = VM.random_small(shape, seed=3)
+    rng = np.random.default_rng(4)
+    f0 = [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+    d = H.DeviceData(w.coef, f0, w.dx, w.dy, w.dz, w.dt)
+    assert d.stream_probe(2) > 0
+    pp, pc, qp, qc = d.fields()
+    d.close()
+    inside = np.zeros(shape, bool)
+    inside[4:-4, 4:-4, 4:-4] = True
+    for got, old, cur in ((pp, f0[0], f0[1]), (qp, f0[2], f0[3])):
+        assert np.array_equal(bits(got[~inside]), bits(old[~inside]))
+        want = 2.0 * cur[inside].astype(np.float64) - old[inside]
+        assert np.max(np.abs(got[inside] - want)) < 1e-5
+    assert np.array_equal(bits(pc), bits(f0[1]))
+    assert np.array_equal(bits(qc), bits(f0[3]))
+
+
 def test_halo_extract_insert_bitwise(H, O, VM):
     shape = (20, 22, 24)
     w = VM.random_small(shape, seed=5)
