@@ -319,38 +319,69 @@ __device__ inline int64_t dev_offset(const LayerGeom &g, int x, int y, int z) {
          (int64_t)(z + STENCIL) * g.zs;
 }
 
-// Corrections of the layer cells from the current fields (before the step).
-// Weights and biases are staged in LDS too (the A operands are read per
-// MFMA); W1 is the slot-ordered copy.
-__global__ void __launch_bounds__(256)
+// Corrections of the layer cells from the current fields (before the step),
+// on v_mfma_f32_16x16x32_f16.  Lane l = 16 kb + col of a wave holds
+//   A[col][8kb + j], B[8kb + j][col] (j = 0..7), C[4kb + r][col] (r = 0..3).
+// A wave runs 16 cells at a time (one per column).  Inputs: K step s gives
+// lane kb the slots 32s + 8kb + j, so K step 0 reads the p arrays and K step
+// 1 the q arrays (slot_feature), and only lanes kb = 3 hold non-patch inputs,
+// in their slots j >= 3: depth one-hot 0..4 (s = 0), one-hot 5..7 and the two
+// Courant numbers (s = 1).  Every input is one LDS read at a per-lane offset
+// fixed for the whole tile: the time differences of the tile's box, the
+// Courant numbers of its cells and a one-hot table are staged side by side.
+// Between layers, C tiles 2s and 2s+1 (units 32s + 4kb + r and 32s + 16 +
+// 4kb + r) are, lane for lane, the B operand of K step s; the next layer's
+// weight columns are permuted to match when they are staged (inf_unit).
+__host__ __device__ constexpr int inf_unit(int s, int p) {  // p = 8kb + j
+  return 32 * s + ((p & 7) < 4 ? 4 * (p >> 3) + (p & 7)
+                               : 16 + 4 * (p >> 3) + (p & 7) - 4);
+}
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// Eight waves per tile: the staging below is shared by twice the waves, and
+// two workgroups (the LDS of two fits a CU) give four waves per SIMD.
+constexpr int kInferThreads = 512;
+
+__global__ void __launch_bounds__(kInferThreads)
     k_layer_infer(const float *pc, const float *pp, const float *qc,
                   const float *qp, LayerGeom g, const SlabTile *__restrict__ tiles,
                   const _Float16 *__restrict__ wh, const float *__restrict__ w32,
                   float2 *__restrict__ delta, int mode, float sigma_max) {
-  // A operands in LDS with rows padded from 16 to 18 h4 (72 halves): lane
-  // (col, kb) reads 8 bytes at row col, so the 64 lanes of a read cover
-  // every bank exactly twice instead of 8 banks eight times.
-  constexpr int kRow = 18;
-  constexpr int lW1 = 0, lW2 = 64 * kRow, lW3 = 128 * kRow;  // h4 offsets
-  __shared__ h4 sW[144 * kRow];
-  __shared__ float sB[kHid + kHid + 16];
-  __shared__ int sDesc[kIn];
-  __shared__ float sD[2][kBoxMax];
+  // Weight rows of 64 halves padded to 72 (144 B): the 16-byte A-operand
+  // reads of 16 rows by 4 lane groups fall on 64 distinct banks.
+  constexpr int kRow = 72;
+  constexpr int lW1 = 0, lW2 = 64 * kRow, lW3 = 128 * kRow;
+  constexpr int kCells = kTile * kTile * kMaxWidth;
+  // Staged inputs, one float array: D of p and of q over the box, the two
+  // Courant numbers of every tile cell, and the one-hot table [d][i].
+  constexpr int xQ = kBoxMax, xCx = 2 * kBoxMax, xCz = xCx + kCells,
+                xOne = xCz + kCells, kX = xOne + kMaxWidth * kMaxWidth;
+  __shared__ __align__(16) _Float16 sW[144 * kRow];
+  __shared__ __align__(16) float sB[kHid + kHid + 16];
+  __shared__ float sX[kX];
+  __shared__ int16_t sL[kCells];  // box index of tile cell e, -1 if none
   const SlabTile T = tiles[blockIdx.x];
-  for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
-    const int row = t >> 4, q = t & 15;
-    sW[lW1 + row * kRow + q] = reinterpret_cast<const h4 *>(wh + hW1p)[t];
-    sW[lW2 + row * kRow + q] = reinterpret_cast<const h4 *>(wh + hW2)[t];
+  const int tid = threadIdx.x;
+  // Weights in runs of 4 halves (8-byte loads): column c..c+3 of a row, c a
+  // multiple of 4, is one run in W1p and, permuted, one run of W2 and W3
+  // (inf_unit maps 4 consecutive columns to 4 consecutive units).
+  typedef _Float16 h4u __attribute__((ext_vector_type(4), aligned(8)));
+  for (int t = tid; t < 144 * 16; t += kInferThreads) {
+    const int row = t >> 4, c = (t & 15) * 4;
+    const int u = inf_unit(c >> 5, c & 31);
+    const h4u w = row < 64 ? *reinterpret_cast<const h4u *>(wh + hW1p + row * 64 + c)
+                : row < 128 ? *reinterpret_cast<const h4u *>(wh + hW2 + (row - 64) * 64 + u)
+                            : *reinterpret_cast<const h4u *>(wh + hW3 + (row - 128) * 64 + u);
+    *reinterpret_cast<h4u *>(&sW[row * kRow + c]) = w;
   }
-  for (int t = threadIdx.x; t < 256; t += blockDim.x)
-    sW[lW3 + (t >> 4) * kRow + (t & 15)] = reinterpret_cast<const h4 *>(wh + hW3)[t];
-  for (int t = threadIdx.x; t < kHid; t += blockDim.x) {
-    sB[t] = w32[oB1 + t];
-    sB[kHid + t] = w32[oB2 + t];
+  if (tid < kHid) {
+    sB[tid] = w32[oB1 + tid];
+    sB[kHid + tid] = w32[oB2 + tid];
   }
-  if (threadIdx.x < 16)
-    sB[2 * kHid + threadIdx.x] = threadIdx.x < kOut ? w32[oB3 + threadIdx.x] : 0.0f;
-  if (threadIdx.x < kIn) sDesc[threadIdx.x] = slot_desc(threadIdx.x);
+  if (tid < 16) sB[2 * kHid + tid] = tid < kOut ? w32[oB3 + tid] : 0.0f;
+  if (tid < kMaxWidth * kMaxWidth)
+    sX[xOne + tid] = (tid >> 3) == (tid & 7) ? 1.0f : 0.0f;
   // The box, x fastest; coordinates outside the arrays are clamped (those
   // box cells are beyond the tile's last cell and never read).
   int lo[3], len[3];
@@ -358,117 +389,153 @@ __global__ void __launch_bounds__(256)
   const int nbox = len[0] * len[1] * len[2];
   const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};  // LDS strides
   const int n[3] = {g.nx, g.ny, g.nz};
-  for (int e = threadIdx.x; e < nbox; e += blockDim.x) {
-    const int bx = e % len[0], by = (e / len[0]) % len[1], bz = e / (len[0] * len[1]);
-    int c[3] = {lo[0] + bx, lo[1] + by, lo[2] + bz};
+  // All of a thread's box loads are issued before the first LDS write.
+  constexpr int kPer = (kBoxMax + kInferThreads - 1) / kInferThreads;
+  float bp[kPer], bq[kPer];
+  int bl[kPer];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) c[a] = min(max(c[a], -STENCIL), n[a] + STENCIL - 1);
-    const uint32_t o = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
-    const int l = bx + by * ls[1] + bz * ls[2];
-    sD[0][l] = pc[o] - pp[o];
-    sD[1][l] = qc[o] - qp[o];
+  for (int k = 0; k < kPer; ++k) {
+    const int e = tid + k * kInferThreads;
+    bl[k] = -1;
+    if (e < nbox) {
+      const int bx = e % len[0], by = (e / len[0]) % len[1], bz = e / (len[0] * len[1]);
+      int c[3] = {lo[0] + bx, lo[1] + by, lo[2] + bz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) c[a] = min(max(c[a], -STENCIL), n[a] + STENCIL - 1);
+      const uint32_t o = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
+      bl[k] = bx + by * ls[1] + bz * ls[2];
+      bp[k] = pc[o] - pp[o];
+      bq[k] = qc[o] - qp[o];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (bl[k] >= 0) {
+      sX[bl[k]] = bp[k];
+      sX[xQ + bl[k]] = bq[k];
+    }
+  // Tile cells: which are layer cells of this face, their box index and
+  // Courant numbers.
+  const int ncell = kTile * kTile * g.width;
+  for (int e = tid; e < ncell; e += kInferThreads) {
+    int c[3];
+    const bool valid = slab_cell(T, e, g, c);
+    float cx = 0.0f, cz = 0.0f;
+    int l = -1;
+    if (valid) {
+      l = (c[0] - lo[0]) + (c[1] - lo[1]) * ls[1] + (c[2] - lo[2]) * ls[2];
+      const uint32_t og = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
+      cx = sqrtf(g.v2px[og]) * g.cdt;
+      cz = sqrtf(g.v2pz[og]) * g.cdt;
+    }
+    sL[e] = (int16_t)l;
+    sX[xCx + e] = cx;
+    sX[xCz + e] = cz;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+  const int lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kb = lane >> 4;
-  int desc[4][4];
-#pragma unroll
-  for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) desc[ks][r] = sDesc[16 * ks + 4 * kb + r];
-  // LDS strides of the tile's frame: outward normal, tangent 1, tangent 2.
+  const bool kb3 = kb == 3;
+  // Per-lane input offsets: slot 32s + 8kb + j.  Patch slots: relative to
+  // the cell's box index, in the tile's frame (outward normal, tangent 1,
+  // tangent 2).  Lanes kb = 3, j >= 3 read the one-hot table or the Courant
+  // numbers instead (below).
   const int axis = T.face >> 1, side = T.face & 1;
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
-  const int ncell = kTile * kTile * g.width;
-  // Every wave takes the same number of groups (ncell is a multiple of 64).
-  for (int g0 = wave * 16; g0 < ncell; g0 += 64) {
+  int off[2][8];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int dd = slot_desc(32 * s + 8 * kb + j);
+      off[s][j] = (dd & 1) ? (s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
+                                 (((dd >> 3) & 3) - 1) * s1 +
+                                 (((dd >> 5) & 3) - 1) * s2
+                           : 0;
+    }
+  // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
+  // lanes of cells outside the layer (their results are not stored).
+  const int lsafe = ls[0] + ls[1] + ls[2];
+  // Every wave takes the same number of groups (ncell is a multiple of 128).
+  for (int g0 = wave * 16; g0 < ncell; g0 += kInferThreads / 4) {
     // An offset the compiler cannot see is zero: it keeps the LDS operand
-    // reads inside the loop instead of holding all 36 in registers.
+    // reads inside the loop instead of holding all 18 in registers.
     int z = 0;
     asm volatile("" : "+s"(z));
     const int e = g0 + col;
-    int c[3];
-    const bool valid = slab_cell(T, e, g, c);
-    const int l = (c[0] - lo[0]) + (c[1] - lo[1]) * ls[1] + (c[2] - lo[2]) * ls[2];
-    const int d = e / (kTile * kTile);
-    float v[4][4];
-    float m = 0.0f;
-    float vx = 0.0f, vz = 0.0f;
-    if (valid) {
-      const uint32_t og = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
-      vx = g.v2px[og];
-      vz = g.v2pz[og];
-    }
+    const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the group's cells, x 8
+    const int lraw = sL[e];
+    const bool valid = lraw >= 0;
+    const int l = valid ? lraw : lsafe;
+    float v[2][8];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
+    for (int s = 0; s < 2; ++s)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int dd = desc[ks][r];
-        float x = 0.0f;
-        if (valid) {
-          if (dd & 1) {
-            const int o = l + (((dd >> 1) & 3) - 1) * sn + (((dd >> 3) & 3) - 1) * s1 +
-                          (((dd >> 5) & 3) - 1) * s2;
-            x = sD[ks < 2 ? 0 : 1][o];
-            m = fmaxf(m, fabsf(x));
-          } else {
-            const int k = dd >> 1;
-            x = k < kCourant ? ((k - kPatch) == d ? 1.0f : 0.0f)
-                             : sqrtf(k == kCourant ? vx : vz) * g.cdt;
-          }
+      for (int j = 0; j < 8; ++j) {
+        int a = l + off[s][j];
+        if (j >= 3) {
+          const int alt = s == 0 ? xOne + d8 + (j - 3)
+                        : j < 6  ? xOne + d8 + (j + 2)
+                        : (j == 6 ? xCx : xCz) + e;
+          a = kb3 ? alt : a;
         }
-        v[ks][r] = x;
+        v[s][j] = sX[a];
       }
+    float m = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        m = fmaxf(m, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
     m = fmaxf(m, __shfl_xor(m, 16));
     m = fmaxf(m, __shfl_xor(m, 32));
     // One reciprocal per cell instead of 54 divisions; the inputs are
     // rounded to half right after, which hides the last-bit difference.
     const float rm = m > 0.0f ? 1.0f / m : 0.0f;
-    h4 X[4];
+    const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
+    h8 X[2];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
+    for (int s = 0; s < 2; ++s)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float x = v[ks][r];
-        X[ks][r] = (_Float16)((desc[ks][r] & 1) ? x * rm : x);
-      }
+      for (int j = 0; j < 8; ++j)
+        X[s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
     h4 H1[4], H2[4];
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
-      f4 acc;
+      f4 acc = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = sB[z + 16 * ot + 4 * kb + r];
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-            sW[z + lW1 + (16 * ot + col) * kRow + 4 * ks + kb], X[ks], acc, 0, 0, 0);
+      for (int s = 0; s < 2; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            *reinterpret_cast<const h8 *>(&sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]),
+            X[s], acc, 0, 0, 0);
       H1[ot] = relu_h(acc);
     }
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
-      f4 acc;
+      f4 acc = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = sB[z + kHid + 16 * ot + 4 * kb + r];
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-            sW[z + lW2 + (16 * ot + col) * kRow + 4 * ks + kb], H1[ks], acc, 0, 0, 0);
+      for (int s = 0; s < 2; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            *reinterpret_cast<const h8 *>(&sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]),
+            __builtin_shufflevector(H1[2 * s], H1[2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+            acc, 0, 0, 0);
       H2[ot] = relu_h(acc);
     }
-    f4 y;
+    f4 y = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = sB[z + 2 * kHid + 4 * kb + r];
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-      y = __builtin_amdgcn_mfma_f32_16x16x16f16(
-          sW[z + lW3 + col * kRow + 4 * ks + kb], H2[ks], y, 0, 0, 0);
+    for (int s = 0; s < 2; ++s)
+      y = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+          *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]),
+          __builtin_shufflevector(H2[2 * s], H2[2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+          y, 0, 0, 0);
     if (lane < 16 && valid) {
       float2 out;
       if (mode == kOutDamping) {
         const float sp = sigma_max / (1.0f + expf(-y[0]));
         const float sq = sigma_max / (1.0f + expf(-y[1]));
-        out = make_float2(-sp * sD[0][l], -sq * sD[1][l]);
+        out = make_float2(-sp * sX[l], -sq * sX[xQ + l]);
       } else {
         out = make_float2(m * y[0], m * y[1]);
       }
@@ -940,7 +1007,7 @@ void layer_infer(State *st, const StepArgs &a) {
     DCX_FAIL("the learned layer runs on FP32 storage");
   const Mlp *m = reinterpret_cast<const Mlp *>(st->layer_mlp);
   const LayerGeom g = geom_of(st, st->layer_w, a.dt * sqrtf(a.dxx));
-  hipLaunchKernelGGL(k_layer_infer, dim3(st->layer_ntiles), dim3(256), 0,
+  hipLaunchKernelGGL(k_layer_infer, dim3(st->layer_ntiles), dim3(kInferThreads), 0,
                      st->stream, st->abi.pc, st->abi.pp, st->abi.qc, st->abi.qp,
                      g, (const SlabTile *)st->layer_tiles, m->wh, m->w32,
                      (float2 *)st->layer_delta, m->out_mode, m->sigma_max);
