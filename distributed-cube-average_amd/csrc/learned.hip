@@ -339,6 +339,20 @@ __host__ __device__ constexpr int inf_unit(int s, int p) {  // p = 8kb + j
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
+// ReLU after the conversion to half, as packed half maxima (rounding keeps
+// the sign, so max(0, round(x)) = round(max(0, x)); a -0 weighs as 0).
+__device__ inline h4 relu_pk(f4 z) {
+  const h4 h = __builtin_convertvector(z, h4);
+  return __builtin_elementwise_max(h, (h4)(_Float16)0.0f);
+}
+
+// floor(e / n) for 0 <= e < 4096 and 1 <= n <= 400 from rn = 1.0f / n: the
+// float quotient is within 4.2e-4 of e / n and a non-integer e / n is at
+// least 1/400 above its floor (checked exhaustively over that range).
+__device__ inline int small_div(int e, float rn) {
+  return (int)((float)e * rn + 1e-3f);
+}
+
 // Eight waves per tile: the staging below is shared by twice the waves, and
 // two workgroups (the LDS of two fits a CU) give four waves per SIMD.
 constexpr int kInferThreads = 512;
@@ -391,6 +405,8 @@ __global__ void __launch_bounds__(kInferThreads)
   const int n[3] = {g.nx, g.ny, g.nz};
   // All of a thread's box loads are issued before the first LDS write.
   constexpr int kPer = (kBoxMax + kInferThreads - 1) / kInferThreads;
+  static_assert(kBoxMax < 4096 && (kTile + 2) * (kTile + 2) <= 400, "small_div range");
+  const float r0 = 1.0f / (float)len[0], r01 = 1.0f / (float)(len[0] * len[1]);
   float bp[kPer], bq[kPer];
   int bl[kPer];
 #pragma unroll
@@ -398,7 +414,8 @@ __global__ void __launch_bounds__(kInferThreads)
     const int e = tid + k * kInferThreads;
     bl[k] = -1;
     if (e < nbox) {
-      const int bx = e % len[0], by = (e / len[0]) % len[1], bz = e / (len[0] * len[1]);
+      const int bz = small_div(e, r01), r = e - bz * len[0] * len[1];
+      const int by = small_div(r, r0), bx = r - by * len[0];
       int c[3] = {lo[0] + bx, lo[1] + by, lo[2] + bz};
 #pragma unroll
       for (int a = 0; a < 3; ++a) c[a] = min(max(c[a], -STENCIL), n[a] + STENCIL - 1);
@@ -444,17 +461,18 @@ __global__ void __launch_bounds__(kInferThreads)
   const int axis = T.face >> 1, side = T.face & 1;
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
-  int off[2][8];
+  int off[2][8];  // bytes
 #pragma unroll
   for (int s = 0; s < 2; ++s)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int dd = slot_desc(32 * s + 8 * kb + j);
-      off[s][j] = (dd & 1) ? (s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
-                                 (((dd >> 3) & 3) - 1) * s1 +
-                                 (((dd >> 5) & 3) - 1) * s2
+      off[s][j] = (dd & 1) ? 4 * ((s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
+                                  (((dd >> 3) & 3) - 1) * s1 +
+                                  (((dd >> 5) & 3) - 1) * s2)
                            : 0;
     }
+  const char *const xb = reinterpret_cast<const char *>(sX);
   // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
   // lanes of cells outside the layer (their results are not stored).
   const int lsafe = ls[0] + ls[1] + ls[2];
@@ -469,19 +487,20 @@ __global__ void __launch_bounds__(kInferThreads)
     const int lraw = sL[e];
     const bool valid = lraw >= 0;
     const int l = valid ? lraw : lsafe;
+    const int lb = 4 * l;
     float v[2][8];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        int a = l + off[s][j];
+        int a = lb + off[s][j];
         if (j >= 3) {
-          const int alt = s == 0 ? xOne + d8 + (j - 3)
-                        : j < 6  ? xOne + d8 + (j + 2)
-                        : (j == 6 ? xCx : xCz) + e;
+          const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
+                             : j < 6  ? xOne + d8 + (j + 2)
+                             : (j == 6 ? xCx : xCz) + e);
           a = kb3 ? alt : a;
         }
-        v[s][j] = sX[a];
+        v[s][j] = *reinterpret_cast<const float *>(xb + a);
       }
     float m = 0.0f;
 #pragma unroll
@@ -493,7 +512,7 @@ __global__ void __launch_bounds__(kInferThreads)
     m = fmaxf(m, __shfl_xor(m, 32));
     // One reciprocal per cell instead of 54 divisions; the inputs are
     // rounded to half right after, which hides the last-bit difference.
-    const float rm = m > 0.0f ? 1.0f / m : 0.0f;
+    const float rm = m > 0.0f ? __builtin_amdgcn_rcpf(m) : 0.0f;
     const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
     h8 X[2];
 #pragma unroll
@@ -510,7 +529,7 @@ __global__ void __launch_bounds__(kInferThreads)
         acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
             *reinterpret_cast<const h8 *>(&sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]),
             X[s], acc, 0, 0, 0);
-      H1[ot] = relu_h(acc);
+      H1[ot] = relu_pk(acc);
     }
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
@@ -521,7 +540,7 @@ __global__ void __launch_bounds__(kInferThreads)
             *reinterpret_cast<const h8 *>(&sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]),
             __builtin_shufflevector(H1[2 * s], H1[2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
             acc, 0, 0, 0);
-      H2[ot] = relu_h(acc);
+      H2[ot] = relu_pk(acc);
     }
     f4 y = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
 #pragma unroll
@@ -533,8 +552,8 @@ __global__ void __launch_bounds__(kInferThreads)
     if (lane < 16 && valid) {
       float2 out;
       if (mode == kOutDamping) {
-        const float sp = sigma_max / (1.0f + expf(-y[0]));
-        const float sq = sigma_max / (1.0f + expf(-y[1]));
+        const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[0]));
+        const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[1]));
         out = make_float2(-sp * sX[l], -sq * sX[xQ + l]);
       } else {
         out = make_float2(m * y[0], m * y[1]);
