@@ -476,89 +476,127 @@ __global__ void __launch_bounds__(kInferThreads)
   // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
   // lanes of cells outside the layer (their results are not stored).
   const int lsafe = ls[0] + ls[1] + ls[2];
-  // Every wave takes the same number of groups (ncell is a multiple of 128).
-  for (int g0 = wave * 16; g0 < ncell; g0 += kInferThreads / 4) {
+  // A wave runs two groups of 16 cells at a time (rows 0-15 and 16-31 of a
+  // 32-cell slice of one depth level): every weight operand read from LDS
+  // feeds two MFMAs, and the two chains interleave.  Every wave takes the
+  // same number of slices (ncell is a multiple of 256).
+  constexpr int G = 2;
+  for (int g0 = wave * 16 * G; g0 < ncell; g0 += kInferThreads / 4 * G) {
     // An offset the compiler cannot see is zero: it keeps the LDS operand
     // reads inside the loop instead of holding all 18 in registers.
     int z = 0;
     asm volatile("" : "+s"(z));
-    const int e = g0 + col;
-    const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the group's cells, x 8
-    const int lraw = sL[e];
-    const bool valid = lraw >= 0;
-    const int l = valid ? lraw : lsafe;
-    const int lb = 4 * l;
-    float v[2][8];
+    const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the slice's cells, x 8
+    int e[G], l[G];
+    bool valid[G];
+    float m[G];
+    h8 X[G][2];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < G; ++q) {
+      e[q] = g0 + 16 * q + col;
+      const int lraw = sL[e[q]];
+      valid[q] = lraw >= 0;
+      l[q] = valid[q] ? lraw : lsafe;
+      const int lb = 4 * l[q];
+      float v[2][8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int a = lb + off[s][j];
-        if (j >= 3) {
-          const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
-                             : j < 6  ? xOne + d8 + (j + 2)
-                             : (j == 6 ? xCx : xCz) + e);
-          a = kb3 ? alt : a;
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int ad = lb + off[s][j];
+          if (j >= 3) {
+            const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
+                               : j < 6  ? xOne + d8 + (j + 2)
+                               : (j == 6 ? xCx : xCz) + e[q]);
+            ad = kb3 ? alt : ad;
+          }
+          v[s][j] = *reinterpret_cast<const float *>(xb + ad);
         }
-        v[s][j] = *reinterpret_cast<const float *>(xb + a);
-      }
-    float m = 0.0f;
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        m = fmaxf(m, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
-    // One reciprocal per cell instead of 54 divisions; the inputs are
-    // rounded to half right after, which hides the last-bit difference.
-    const float rm = m > 0.0f ? __builtin_amdgcn_rcpf(m) : 0.0f;
-    const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
-    h8 X[2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        X[s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
-    h4 H1[4], H2[4];
-#pragma unroll
-    for (int ot = 0; ot < 4; ++ot) {
-      f4 acc = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
+      float mm = 0.0f;
 #pragma unroll
       for (int s = 0; s < 2; ++s)
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-            *reinterpret_cast<const h8 *>(&sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]),
-            X[s], acc, 0, 0, 0);
-      H1[ot] = relu_pk(acc);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          mm = fmaxf(mm, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
+      mm = fmaxf(mm, __shfl_xor(mm, 16));
+      mm = fmaxf(mm, __shfl_xor(mm, 32));
+      m[q] = mm;
+      // One reciprocal per cell instead of 54 divisions; the inputs are
+      // rounded to half right after, which hides the last-bit difference.
+      const float rm = mm > 0.0f ? __builtin_amdgcn_rcpf(mm) : 0.0f;
+      const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          X[q][s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
+    }
+    h4 H1[G][4], H2[G][4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
+      f4 acc[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) acc[q] = bias;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const h8 w = *reinterpret_cast<const h8 *>(
+            &sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, X[q][s], acc[q], 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q) H1[q][ot] = relu_pk(acc[q]);
     }
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
-      f4 acc = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
+      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
+      f4 acc[G];
 #pragma unroll
-      for (int s = 0; s < 2; ++s)
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-            *reinterpret_cast<const h8 *>(&sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]),
-            __builtin_shufflevector(H1[2 * s], H1[2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
-            acc, 0, 0, 0);
-      H2[ot] = relu_pk(acc);
-    }
-    f4 y = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
+      for (int q = 0; q < G; ++q) acc[q] = bias;
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-      y = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-          *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]),
-          __builtin_shufflevector(H2[2 * s], H2[2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
-          y, 0, 0, 0);
-    if (lane < 16 && valid) {
-      float2 out;
-      if (mode == kOutDamping) {
-        const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[0]));
-        const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[1]));
-        out = make_float2(-sp * sX[l], -sq * sX[xQ + l]);
-      } else {
-        out = make_float2(m * y[0], m * y[1]);
+      for (int s = 0; s < 2; ++s) {
+        const h8 w = *reinterpret_cast<const h8 *>(
+            &sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+              w, __builtin_shufflevector(H1[q][2 * s], H1[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+              acc[q], 0, 0, 0);
       }
-      delta[(size_t)blockIdx.x * ncell + e] = out;
+#pragma unroll
+      for (int q = 0; q < G; ++q) H2[q][ot] = relu_pk(acc[q]);
+    }
+    f4 y[G];
+    {
+      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
+#pragma unroll
+      for (int q = 0; q < G; ++q) y[q] = bias;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const h8 w = *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          y[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+              w, __builtin_shufflevector(H2[q][2 * s], H2[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+              y[q], 0, 0, 0);
+      }
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        if (!valid[q]) continue;
+        float2 out;
+        if (mode == kOutDamping) {
+          const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][0]));
+          const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][1]));
+          out = make_float2(-sp * sX[l[q]], -sq * sX[xQ + l[q]]);
+        } else {
+          out = make_float2(m[q] * y[q][0], m[q] * y[q][1]);
+        }
+        delta[(size_t)blockIdx.x * ncell + e[q]] = out;
+      }
     }
   }
 }
