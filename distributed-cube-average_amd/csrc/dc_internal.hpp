@@ -168,6 +168,11 @@ struct State {
   int layer_w, layer_n, layer_ntiles;
   void *layer_tiles;   // 16 x 16 face-plane tiles of the layer
   float *layer_delta;  // corrections, per tile cell
+  // Per tile cell, fixed while the layer is set: the cell's index in its
+  // tile's staged box (-1 where the cell is not a layer cell of that face)
+  // and sqrt(v2px), sqrt(v2pz) there.
+  int16_t *layer_lidx;
+  float2 *layer_root;
   // Layer cells used when the struct is the small domain of
   // dcx_learned_sample_step.
   int sample_w, sample_n;

@@ -353,20 +353,51 @@ __device__ inline int small_div(int e, float rn) {
   return (int)((float)e * rn + 1e-3f);
 }
 
-// Eight waves per tile: the staging below is shared by twice the waves, and
-// two workgroups (the LDS of two fits a CU) give four waves per SIMD.
+// Eight waves per tile, two workgroups per CU (their LDS fits): four waves
+// per SIMD.
 constexpr int kInferThreads = 512;
+constexpr int kInferCells = kTile * kTile * kMaxWidth;  // tile cells, widest layer
 
+// Static per-cell records of every tile (set once per layer): the cell's
+// index in its tile's staged box, -1 if it is not a layer cell of the tile's
+// face, and sqrt(v2px), sqrt(v2pz) there.
+__global__ void __launch_bounds__(256)
+    k_layer_static(LayerGeom g, const SlabTile *__restrict__ tiles,
+                   int16_t *__restrict__ lidx, float2 *__restrict__ root) {
+  const SlabTile T = tiles[blockIdx.x];
+  int lo[3], len[3];
+  slab_box(T, g, lo, len);
+  const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};
+  const int ncell = kTile * kTile * g.width;
+  for (int e = threadIdx.x; e < ncell; e += blockDim.x) {
+    int c[3];
+    int l = -1;
+    float2 r = make_float2(0.0f, 0.0f);
+    if (slab_cell(T, e, g, c)) {
+      l = (c[0] - lo[0]) + (c[1] - lo[1]) * ls[1] + (c[2] - lo[2]) * ls[2];
+      const uint32_t og = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
+      r = make_float2(sqrtf(g.v2px[og]), sqrtf(g.v2pz[og]));
+    }
+    lidx[(size_t)blockIdx.x * ncell + e] = (int16_t)l;
+    root[(size_t)blockIdx.x * ncell + e] = r;
+  }
+}
+
+// One workgroup per tile, two per CU: the weights, the tile's box of time
+// differences and its cells' static records are staged in LDS, then the
+// tile's cells run through the network.
 __global__ void __launch_bounds__(kInferThreads)
     k_layer_infer(const float *pc, const float *pp, const float *qc,
                   const float *qp, LayerGeom g, const SlabTile *__restrict__ tiles,
+                  int ntiles, const int16_t *__restrict__ lidx,
+                  const float2 *__restrict__ root,
                   const _Float16 *__restrict__ wh, const float *__restrict__ w32,
                   float2 *__restrict__ delta, int mode, float sigma_max) {
   // Weight rows of 64 halves padded to 72 (144 B): the 16-byte A-operand
   // reads of 16 rows by 4 lane groups fall on 64 distinct banks.
   constexpr int kRow = 72;
   constexpr int lW1 = 0, lW2 = 64 * kRow, lW3 = 128 * kRow;
-  constexpr int kCells = kTile * kTile * kMaxWidth;
+  constexpr int kCells = kInferCells;
   // Staged inputs, one float array: D of p and of q over the box, the two
   // Courant numbers of every tile cell, and the one-hot table [d][i].
   constexpr int xQ = kBoxMax, xCx = 2 * kBoxMax, xCz = xCx + kCells,
@@ -375,7 +406,6 @@ __global__ void __launch_bounds__(kInferThreads)
   __shared__ __align__(16) float sB[kHid + kHid + 16];
   __shared__ float sX[kX];
   __shared__ int16_t sL[kCells];  // box index of tile cell e, -1 if none
-  const SlabTile T = tiles[blockIdx.x];
   const int tid = threadIdx.x;
   // Weights in runs of 4 halves (8-byte loads): column c..c+3 of a row, c a
   // multiple of 4, is one run in W1p and, permuted, one run of W2 and W3
@@ -396,206 +426,245 @@ __global__ void __launch_bounds__(kInferThreads)
   if (tid < 16) sB[2 * kHid + tid] = tid < kOut ? w32[oB3 + tid] : 0.0f;
   if (tid < kMaxWidth * kMaxWidth)
     sX[xOne + tid] = (tid >> 3) == (tid & 7) ? 1.0f : 0.0f;
-  // The box, x fastest; coordinates outside the arrays are clamped (those
-  // box cells are beyond the tile's last cell and never read).
-  int lo[3], len[3];
-  slab_box(T, g, lo, len);
-  const int nbox = len[0] * len[1] * len[2];
-  const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};  // LDS strides
-  const int n[3] = {g.nx, g.ny, g.nz};
-  // All of a thread's box loads are issued before the first LDS write.
-  constexpr int kPer = (kBoxMax + kInferThreads - 1) / kInferThreads;
-  static_assert(kBoxMax < 4096 && (kTile + 2) * (kTile + 2) <= 400, "small_div range");
-  const float r0 = 1.0f / (float)len[0], r01 = 1.0f / (float)(len[0] * len[1]);
-  float bp[kPer], bq[kPer];
-  int bl[kPer];
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int e = tid + k * kInferThreads;
-    bl[k] = -1;
-    if (e < nbox) {
-      const int bz = small_div(e, r01), r = e - bz * len[0] * len[1];
-      const int by = small_div(r, r0), bx = r - by * len[0];
-      int c[3] = {lo[0] + bx, lo[1] + by, lo[2] + bz};
-#pragma unroll
-      for (int a = 0; a < 3; ++a) c[a] = min(max(c[a], -STENCIL), n[a] + STENCIL - 1);
-      const uint32_t o = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
-      bl[k] = bx + by * ls[1] + bz * ls[2];
-      bp[k] = pc[o] - pp[o];
-      bq[k] = qc[o] - qp[o];
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kPer; ++k)
-    if (bl[k] >= 0) {
-      sX[bl[k]] = bp[k];
-      sX[xQ + bl[k]] = bq[k];
-    }
-  // Tile cells: which are layer cells of this face, their box index and
-  // Courant numbers.
-  const int ncell = kTile * kTile * g.width;
-  for (int e = tid; e < ncell; e += kInferThreads) {
-    int c[3];
-    const bool valid = slab_cell(T, e, g, c);
-    float cx = 0.0f, cz = 0.0f;
-    int l = -1;
-    if (valid) {
-      l = (c[0] - lo[0]) + (c[1] - lo[1]) * ls[1] + (c[2] - lo[2]) * ls[2];
-      const uint32_t og = (uint32_t)dev_offset(g, c[0], c[1], c[2]);
-      cx = sqrtf(g.v2px[og]) * g.cdt;
-      cz = sqrtf(g.v2pz[og]) * g.cdt;
-    }
-    sL[e] = (int16_t)l;
-    sX[xCx + e] = cx;
-    sX[xCz + e] = cz;
-  }
-  __syncthreads();
 
-  const int lane = tid & 63, wave = tid >> 6;
-  const int col = lane & 15, kb = lane >> 4;
-  const bool kb3 = kb == 3;
-  // Per-lane input offsets: slot 32s + 8kb + j.  Patch slots: relative to
-  // the cell's box index, in the tile's frame (outward normal, tangent 1,
-  // tangent 2).  Lanes kb = 3, j >= 3 read the one-hot table or the Courant
-  // numbers instead (below).
-  const int axis = T.face >> 1, side = T.face & 1;
-  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
-  const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
-  int off[2][8];  // bytes
+  const int ncell = kTile * kTile * g.width;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  // One tile's loads, held in registers until its turn: every box cell's
+  // time differences (all issued before any is used) and the static records
+  // of the thread's tile cells.
+  constexpr int kPer = (kBoxMax + kInferThreads - 1) / kInferThreads;
+  constexpr int kCPer = kCells / kInferThreads;
+  static_assert(kBoxMax < 4096 && (kTile + 2) * (kTile + 2) <= 400, "small_div range");
+  float bp[kPer], bq[kPer];
+  int16_t cl[kCPer];
+  float2 cr[kCPer];
+  // Box cell e of a tile: its (x fastest) box coordinates and LDS index.
+  struct BoxGeom {
+    int lo0, lo1, lo2, len0, len1, nbox, ls1, ls2;
+    float r0, r01;
+  };
+  auto box_geom = [&](const SlabTile &T) {
+    int lo[3], len[3];
+    slab_box(T, g, lo, len);
+    BoxGeom b;
+    b.lo0 = lo[0]; b.lo1 = lo[1]; b.lo2 = lo[2];
+    b.len0 = len[0]; b.len1 = len[1];
+    b.nbox = len[0] * len[1] * len[2];
+    b.ls1 = len[0] | 1;
+    b.ls2 = (len[0] | 1) * len[1];
+    b.r0 = 1.0f / (float)len[0];
+    b.r01 = 1.0f / (float)(len[0] * len[1]);
+    return b;
+  };
+  auto fetch = [&](int t) {
+    const BoxGeom b = box_geom(tiles[t]);
 #pragma unroll
-  for (int s = 0; s < 2; ++s)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int dd = slot_desc(32 * s + 8 * kb + j);
-      off[s][j] = (dd & 1) ? 4 * ((s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
-                                  (((dd >> 3) & 3) - 1) * s1 +
-                                  (((dd >> 5) & 3) - 1) * s2)
-                           : 0;
-    }
-  const char *const xb = reinterpret_cast<const char *>(sX);
-  // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
-  // lanes of cells outside the layer (their results are not stored).
-  const int lsafe = ls[0] + ls[1] + ls[2];
-  // A wave runs two groups of 16 cells at a time (rows 0-15 and 16-31 of a
-  // 32-cell slice of one depth level): every weight operand read from LDS
-  // feeds two MFMAs, and the two chains interleave.  Every wave takes the
-  // same number of slices (ncell is a multiple of 256).
-  constexpr int G = 2;
-  for (int g0 = wave * 16 * G; g0 < ncell; g0 += kInferThreads / 4 * G) {
-    // An offset the compiler cannot see is zero: it keeps the LDS operand
-    // reads inside the loop instead of holding all 18 in registers.
-    int z = 0;
-    asm volatile("" : "+s"(z));
-    const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the slice's cells, x 8
-    int e[G], l[G];
-    bool valid[G];
-    float m[G];
-    h8 X[G][2];
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      e[q] = g0 + 16 * q + col;
-      const int lraw = sL[e[q]];
-      valid[q] = lraw >= 0;
-      l[q] = valid[q] ? lraw : lsafe;
-      const int lb = 4 * l[q];
-      float v[2][8];
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          int ad = lb + off[s][j];
-          if (j >= 3) {
-            const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
-                               : j < 6  ? xOne + d8 + (j + 2)
-                               : (j == 6 ? xCx : xCz) + e[q]);
-            ad = kb3 ? alt : ad;
-          }
-          v[s][j] = *reinterpret_cast<const float *>(xb + ad);
-        }
-      float mm = 0.0f;
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          mm = fmaxf(mm, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
-      mm = fmaxf(mm, __shfl_xor(mm, 16));
-      mm = fmaxf(mm, __shfl_xor(mm, 32));
-      m[q] = mm;
-      // One reciprocal per cell instead of 54 divisions; the inputs are
-      // rounded to half right after, which hides the last-bit difference.
-      const float rm = mm > 0.0f ? __builtin_amdgcn_rcpf(mm) : 0.0f;
-      const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          X[q][s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
-    }
-    h4 H1[G][4], H2[G][4];
-#pragma unroll
-    for (int ot = 0; ot < 4; ++ot) {
-      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
-      f4 acc[G];
-#pragma unroll
-      for (int q = 0; q < G; ++q) acc[q] = bias;
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const h8 w = *reinterpret_cast<const h8 *>(
-            &sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
-#pragma unroll
-        for (int q = 0; q < G; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, X[q][s], acc[q], 0, 0, 0);
+    for (int k = 0; k < kPer; ++k) {
+      const int e = tid + k * kInferThreads;
+      bp[k] = bq[k] = 0.0f;
+      if (e < b.nbox) {
+        const int bz = small_div(e, b.r01), r = e - bz * b.len0 * b.len1;
+        const int by = small_div(r, b.r0), bx = r - by * b.len0;
+        const int cx = min(max(b.lo0 + bx, -STENCIL), n[0] + STENCIL - 1);
+        const int cy = min(max(b.lo1 + by, -STENCIL), n[1] + STENCIL - 1);
+        const int cz = min(max(b.lo2 + bz, -STENCIL), n[2] + STENCIL - 1);
+        const uint32_t o = (uint32_t)dev_offset(g, cx, cy, cz);
+        bp[k] = pc[o] - pp[o];
+        bq[k] = qc[o] - qp[o];
       }
-#pragma unroll
-      for (int q = 0; q < G; ++q) H1[q][ot] = relu_pk(acc[q]);
     }
 #pragma unroll
-    for (int ot = 0; ot < 4; ++ot) {
-      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
-      f4 acc[G];
-#pragma unroll
-      for (int q = 0; q < G; ++q) acc[q] = bias;
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const h8 w = *reinterpret_cast<const h8 *>(
-            &sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
-#pragma unroll
-        for (int q = 0; q < G; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-              w, __builtin_shufflevector(H1[q][2 * s], H1[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
-              acc[q], 0, 0, 0);
+    for (int k = 0; k < kCPer; ++k) {
+      const int e = tid + k * kInferThreads;
+      cl[k] = -1;
+      cr[k] = make_float2(0.0f, 0.0f);
+      if (e < ncell) {
+        cl[k] = lidx[(size_t)t * ncell + e];
+        cr[k] = root[(size_t)t * ncell + e];
       }
-#pragma unroll
-      for (int q = 0; q < G; ++q) H2[q][ot] = relu_pk(acc[q]);
     }
-    f4 y[G];
+  };
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  {
+    // All of the tile's loads are issued before the first LDS write; the
+    // other workgroup on the CU computes meanwhile.
+    fetch(t);
     {
-      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
+      const BoxGeom b = box_geom(tiles[t]);
 #pragma unroll
-      for (int q = 0; q < G; ++q) y[q] = bias;
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const h8 w = *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]);
-#pragma unroll
-        for (int q = 0; q < G; ++q)
-          y[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-              w, __builtin_shufflevector(H2[q][2 * s], H2[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
-              y[q], 0, 0, 0);
+      for (int k = 0; k < kPer; ++k) {
+        const int e = tid + k * kInferThreads;
+        if (e < b.nbox) {
+          const int bz = small_div(e, b.r01), r = e - bz * b.len0 * b.len1;
+          const int by = small_div(r, b.r0), bx = r - by * b.len0;
+          const int l = bx + by * b.ls1 + bz * b.ls2;
+          sX[l] = bp[k];
+          sX[xQ + l] = bq[k];
+        }
       }
     }
-    if (lane < 16) {
+#pragma unroll
+    for (int k = 0; k < kCPer; ++k) {
+      const int e = tid + k * kInferThreads;
+      if (e < ncell) {
+        sL[e] = cl[k];
+        sX[xCx + e] = cr[k].x * g.cdt;
+        sX[xCz + e] = cr[k].y * g.cdt;
+      }
+    }
+    __syncthreads();
+    const SlabTile T = tiles[t];
+    int lo[3], len[3];
+    slab_box(T, g, lo, len);
+    const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};  // LDS strides
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, kb = lane >> 4;
+    const bool kb3 = kb == 3;
+    // Per-lane input offsets: slot 32s + 8kb + j.  Patch slots: relative to
+    // the cell's box index, in the tile's frame (outward normal, tangent 1,
+    // tangent 2).  Lanes kb = 3, j >= 3 read the one-hot table or the Courant
+    // numbers instead (below).
+    const int axis = T.face >> 1, side = T.face & 1;
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
+    int off[2][8];  // bytes
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int dd = slot_desc(32 * s + 8 * kb + j);
+        off[s][j] = (dd & 1) ? 4 * ((s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
+                                    (((dd >> 3) & 3) - 1) * s1 +
+                                    (((dd >> 5) & 3) - 1) * s2)
+                             : 0;
+      }
+    const char *const xb = reinterpret_cast<const char *>(sX);
+    // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
+    // lanes of cells outside the layer (their results are not stored).
+    const int lsafe = ls[0] + ls[1] + ls[2];
+    // A wave runs two groups of 16 cells at a time (rows 0-15 and 16-31 of a
+    // 32-cell slice of one depth level): every weight operand read from LDS
+    // feeds two MFMAs, and the two chains interleave.  Every wave takes the
+    // same number of slices (ncell is a multiple of 256).
+    constexpr int G = 2;
+    for (int g0 = wave * 16 * G; g0 < ncell; g0 += kInferThreads / 4 * G) {
+      // An offset the compiler cannot see is zero: it keeps the LDS operand
+      // reads inside the loop instead of holding all 18 in registers.
+      int z = 0;
+      asm volatile("" : "+s"(z));
+      const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the slice's cells, x 8
+      int e[G], l[G];
+      bool valid[G];
+      float m[G];
+      h8 X[G][2];
 #pragma unroll
       for (int q = 0; q < G; ++q) {
-        if (!valid[q]) continue;
-        float2 out;
-        if (mode == kOutDamping) {
-          const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][0]));
-          const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][1]));
-          out = make_float2(-sp * sX[l[q]], -sq * sX[xQ + l[q]]);
-        } else {
-          out = make_float2(m[q] * y[q][0], m[q] * y[q][1]);
+        e[q] = g0 + 16 * q + col;
+        const int lraw = sL[e[q]];
+        valid[q] = lraw >= 0;
+        l[q] = valid[q] ? lraw : lsafe;
+        const int lb = 4 * l[q];
+        float v[2][8];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            int ad = lb + off[s][j];
+            if (j >= 3) {
+              const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
+                                 : j < 6  ? xOne + d8 + (j + 2)
+                                 : (j == 6 ? xCx : xCz) + e[q]);
+              ad = kb3 ? alt : ad;
+            }
+            v[s][j] = *reinterpret_cast<const float *>(xb + ad);
+          }
+        float mm = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            mm = fmaxf(mm, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
+        mm = fmaxf(mm, __shfl_xor(mm, 16));
+        mm = fmaxf(mm, __shfl_xor(mm, 32));
+        m[q] = mm;
+        // One reciprocal per cell instead of 54 divisions; the inputs are
+        // rounded to half right after, which hides the last-bit difference.
+        const float rm = mm > 0.0f ? __builtin_amdgcn_rcpf(mm) : 0.0f;
+        const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            X[q][s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
+      }
+      h4 H1[G][4], H2[G][4];
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) {
+        const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
+        f4 acc[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q] = bias;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const h8 w = *reinterpret_cast<const h8 *>(
+              &sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+          for (int q = 0; q < G; ++q)
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, X[q][s], acc[q], 0, 0, 0);
         }
-        delta[(size_t)blockIdx.x * ncell + e[q]] = out;
+#pragma unroll
+        for (int q = 0; q < G; ++q) H1[q][ot] = relu_pk(acc[q]);
+      }
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) {
+        const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
+        f4 acc[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q] = bias;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const h8 w = *reinterpret_cast<const h8 *>(
+              &sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+          for (int q = 0; q < G; ++q)
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                w, __builtin_shufflevector(H1[q][2 * s], H1[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+                acc[q], 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) H2[q][ot] = relu_pk(acc[q]);
+      }
+      f4 y[G];
+      {
+        const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q) y[q] = bias;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const h8 w = *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+          for (int q = 0; q < G; ++q)
+            y[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                w, __builtin_shufflevector(H2[q][2 * s], H2[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+                y[q], 0, 0, 0);
+        }
+      }
+      if (lane < 16) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          if (!valid[q]) continue;
+          float2 out;
+          if (mode == kOutDamping) {
+            const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][0]));
+            const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][1]));
+            out = make_float2(-sp * sX[l[q]], -sq * sX[xQ + l[q]]);
+          } else {
+            out = make_float2(m[q] * y[q][0], m[q] * y[q][1]);
+          }
+          delta[(size_t)t * ncell + e[q]] = out;
+        }
       }
     }
   }
@@ -1066,7 +1135,8 @@ void layer_infer(State *st, const StepArgs &a) {
   const LayerGeom g = geom_of(st, st->layer_w, a.dt * sqrtf(a.dxx));
   hipLaunchKernelGGL(k_layer_infer, dim3(st->layer_ntiles), dim3(kInferThreads), 0,
                      st->stream, st->abi.pc, st->abi.pp, st->abi.qc, st->abi.qp,
-                     g, (const SlabTile *)st->layer_tiles, m->wh, m->w32,
+                     g, (const SlabTile *)st->layer_tiles, st->layer_ntiles,
+                     st->layer_lidx, st->layer_root, m->wh, m->w32,
                      (float2 *)st->layer_delta, m->out_mode, m->sigma_max);
   DCX_CHECK(hipGetLastError());
 }
@@ -1089,6 +1159,10 @@ void layer_free(State *st) {
   mlp_attach(st->layer_mlp, -1);
   if (st->layer_tiles) DCX_CHECK(hipFree(st->layer_tiles));
   if (st->layer_delta) DCX_CHECK(hipFree(st->layer_delta));
+  if (st->layer_lidx) DCX_CHECK(hipFree(st->layer_lidx));
+  if (st->layer_root) DCX_CHECK(hipFree(st->layer_root));
+  st->layer_lidx = nullptr;
+  st->layer_root = nullptr;
   if (st->sample_cells) DCX_CHECK(hipFree(st->sample_cells));
   st->layer_tiles = nullptr;
   st->layer_delta = nullptr;
@@ -1321,8 +1395,12 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *p, int width) {
   DCX_CHECK(hipStreamSynchronize(st->stream));
   if (st->layer_tiles) DCX_CHECK(hipFree(st->layer_tiles));
   if (st->layer_delta) DCX_CHECK(hipFree(st->layer_delta));
+  if (st->layer_lidx) DCX_CHECK(hipFree(st->layer_lidx));
+  if (st->layer_root) DCX_CHECK(hipFree(st->layer_root));
   st->layer_tiles = nullptr;
   st->layer_delta = nullptr;
+  st->layer_lidx = nullptr;
+  st->layer_root = nullptr;
   st->layer_n = 0;
   st->layer_ntiles = 0;
   mlp_attach(st->layer_mlp, -1);
@@ -1348,8 +1426,19 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *p, int width) {
   DCX_CHECK(hipMalloc(&st->layer_tiles, tiles.size() * sizeof(SlabTile)));
   DCX_CHECK(hipMemcpy(st->layer_tiles, tiles.data(), tiles.size() * sizeof(SlabTile),
                       hipMemcpyHostToDevice));
-  DCX_CHECK(hipMalloc(&st->layer_delta,
-                      tiles.size() * (size_t)kTile * kTile * width * 2 * sizeof(float)));
+  const size_t ncells = tiles.size() * (size_t)kTile * kTile * width;
+  DCX_CHECK(hipMalloc(&st->layer_delta, ncells * 2 * sizeof(float)));
+  DCX_CHECK(hipMalloc(&st->layer_lidx, ncells * sizeof(int16_t)));
+  DCX_CHECK(hipMalloc(&st->layer_root, ncells * sizeof(float2)));
+  st->layer_w = width;
+  {
+    const LayerGeom g = geom_of(st, width);
+    hipLaunchKernelGGL(k_layer_static, dim3((unsigned)tiles.size()), dim3(256), 0,
+                       st->stream, g, (const SlabTile *)st->layer_tiles,
+                       st->layer_lidx, st->layer_root);
+    DCX_CHECK(hipGetLastError());
+    DCX_CHECK(hipStreamSynchronize(st->stream));
+  }
   int64_t inner = 1;
   for (int a = 0; a < 3; ++a) inner *= n[a] - 2 * width;
   st->layer_n = (int)(n[0] * n[1] * n[2] - inner);
