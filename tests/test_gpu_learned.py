@@ -209,15 +209,17 @@ def test_sample_step_matches_oracle(H, O, VM, tapered):
         obj.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_layer_step_matches_oracle(H, O, VM, mode):
+@pytest.mark.parametrize("mode,width", [(0, 5), (1, 5), (1, 1), (1, 8)])
+def test_layer_step_matches_oracle(H, O, VM, mode, width):
+    """One step with the layer against the oracle's step plus the oracle's
+    network corrections.  Widths 1 and 8 (the widest the inference kernel's
+    LDS is sized for) cover the smallest and largest tile boxes."""
     rng = np.random.default_rng(9)
     w = VM.random_small((28, 30, 34), seed=4)
     f = _fields(rng, w.shape)
     S = 4
     for a in f:  # the frame is zero in a domain with nothing beyond its edge
         a[:S], a[-S:], a[:, :S], a[:, -S:], a[:, :, :S], a[:, :, -S:] = 0, 0, 0, 0, 0, 0
-    width = 5
     params = rand_params(10, out_scale=0.05)
     mlp = H.Mlp(params, mode=mode, sigma_max=0.5 if mode else 0.0)
     d = H.DeviceData(w.coef, [a.copy() for a in f], w.dx, w.dy, w.dz, w.dt,
