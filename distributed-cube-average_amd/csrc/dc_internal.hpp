@@ -138,7 +138,9 @@ struct State {
   float *stage;  // device staging for the ABI halo copies
   size_t stage_n;
   int kernel;
-  int tile_x, tile_y, zchunk;
+  int tile_x, tile_y;
+  int zchunk;  // planes per workgroup of the fast kernel; 0 = chosen per launch
+  int n_cu;    // compute units of the struct's device
   int xcd_remap;
   int deep;
   int timing_detail;

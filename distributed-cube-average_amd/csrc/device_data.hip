@@ -37,6 +37,22 @@ static float *field_of(const State *st, const float *token) {
   DCX_FAIL("array is not one of this dc_device_data's fields");
 }
 
+// z chunk of a fast launch when the struct leaves it to the library: the
+// longest chunk (at most 512 planes) that still gives the launch two tiles
+// per compute unit, and never below 32 planes (each chunk re-reads 8 planes
+// to fill its z rings).  A 512^3 box keeps whole 512-plane columns; a 200^3
+// shot, 91 columns of 32 x 16 cells, gets 34-plane chunks.
+static int auto_zchunk(const State *st, const Box &box) {
+  const int64_t nx = box.x1 - box.x0, ny = box.y1 - box.y0, nz = box.z1 - box.z0;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 512;
+  const int64_t cols = ((nx + st->tile_x - 1) / st->tile_x) *
+                       ((ny + st->tile_y - 1) / st->tile_y);
+  const int64_t target = 2 * (int64_t)(st->n_cu > 0 ? st->n_cu : 256);
+  const int64_t chunks = (target + cols - 1) / cols;
+  const int64_t z = (nz + chunks - 1) / chunks;
+  return (int)(z < 32 ? 32 : z > 512 ? 512 : z);
+}
+
 StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
                         float dz, float dt) {
   StepArgs a{};
@@ -63,7 +79,7 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   a.src_z = -1;
   a.src_xy = -1;
   a.src_amp = 0.0f;
-  a.zchunk = st->zchunk;
+  a.zchunk = st->zchunk > 0 ? st->zchunk : auto_zchunk(st, box);
   a.xcd_remap = st->xcd_remap;
   a.deep = st->deep;
   a.fstore = st->fstore;
@@ -304,7 +320,9 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   st->kernel = DCX_KERNEL_FAST;
   st->tile_x = 32;
   st->tile_y = 16;
-  st->zchunk = 512;
+  st->zchunk = 0;
+  DCX_CHECK(hipDeviceGetAttribute(&st->n_cu, hipDeviceAttributeMultiprocessorCount,
+                                  st->device));
   st->xcd_remap = 0;
   st->deep = 1;
   const size_t abytes = (size_t)st->alloc_floats * sizeof(float);
@@ -478,7 +496,7 @@ int dcx_set_kernel(dc_device_data *data, int kernel) {
 
 int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk) {
   State *st = state_of(data);
-  if (!fast_tiling_supported(tile_x, tile_y) || z_chunk < 1) return -1;
+  if (!fast_tiling_supported(tile_x, tile_y) || z_chunk < 0) return -1;
   st->tile_x = tile_x;
   st->tile_y = tile_y;
   st->zchunk = z_chunk;
