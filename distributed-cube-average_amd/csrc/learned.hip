@@ -427,244 +427,214 @@ __global__ void __launch_bounds__(kInferThreads)
   if (tid < kMaxWidth * kMaxWidth)
     sX[xOne + tid] = (tid >> 3) == (tid & 7) ? 1.0f : 0.0f;
 
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const SlabTile T = tiles[t];
   const int ncell = kTile * kTile * g.width;
   const int n[3] = {g.nx, g.ny, g.nz};
-  // One tile's loads, held in registers until its turn: every box cell's
-  // time differences (all issued before any is used) and the static records
-  // of the thread's tile cells.
+  int lo[3], len[3];
+  slab_box(T, g, lo, len);
+  const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};  // LDS strides
+  const int nbox = len[0] * len[1] * len[2];
+  // Every load of the tile (each box cell's time differences, the static
+  // records of the thread's tile cells) is issued before the first LDS
+  // write; the other workgroup on the CU computes meanwhile.
   constexpr int kPer = (kBoxMax + kInferThreads - 1) / kInferThreads;
   constexpr int kCPer = kCells / kInferThreads;
   static_assert(kBoxMax < 4096 && (kTile + 2) * (kTile + 2) <= 400, "small_div range");
+  const float r0 = 1.0f / (float)len[0], r01 = 1.0f / (float)(len[0] * len[1]);
   float bp[kPer], bq[kPer];
+  int bl[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = tid + k * kInferThreads;
+    bl[k] = -1;
+    if (e < nbox) {
+      const int bz = small_div(e, r01), r = e - bz * len[0] * len[1];
+      const int by = small_div(r, r0), bx = r - by * len[0];
+      const int cx = min(max(lo[0] + bx, -STENCIL), n[0] + STENCIL - 1);
+      const int cy = min(max(lo[1] + by, -STENCIL), n[1] + STENCIL - 1);
+      const int cz = min(max(lo[2] + bz, -STENCIL), n[2] + STENCIL - 1);
+      const uint32_t o = (uint32_t)dev_offset(g, cx, cy, cz);
+      bl[k] = bx + by * ls[1] + bz * ls[2];
+      bp[k] = pc[o] - pp[o];
+      bq[k] = qc[o] - qp[o];
+    }
+  }
   int16_t cl[kCPer];
   float2 cr[kCPer];
-  // Box cell e of a tile: its (x fastest) box coordinates and LDS index.
-  struct BoxGeom {
-    int lo0, lo1, lo2, len0, len1, nbox, ls1, ls2;
-    float r0, r01;
-  };
-  auto box_geom = [&](const SlabTile &T) {
-    int lo[3], len[3];
-    slab_box(T, g, lo, len);
-    BoxGeom b;
-    b.lo0 = lo[0]; b.lo1 = lo[1]; b.lo2 = lo[2];
-    b.len0 = len[0]; b.len1 = len[1];
-    b.nbox = len[0] * len[1] * len[2];
-    b.ls1 = len[0] | 1;
-    b.ls2 = (len[0] | 1) * len[1];
-    b.r0 = 1.0f / (float)len[0];
-    b.r01 = 1.0f / (float)(len[0] * len[1]);
-    return b;
-  };
-  auto fetch = [&](int t) {
-    const BoxGeom b = box_geom(tiles[t]);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int e = tid + k * kInferThreads;
-      bp[k] = bq[k] = 0.0f;
-      if (e < b.nbox) {
-        const int bz = small_div(e, b.r01), r = e - bz * b.len0 * b.len1;
-        const int by = small_div(r, b.r0), bx = r - by * b.len0;
-        const int cx = min(max(b.lo0 + bx, -STENCIL), n[0] + STENCIL - 1);
-        const int cy = min(max(b.lo1 + by, -STENCIL), n[1] + STENCIL - 1);
-        const int cz = min(max(b.lo2 + bz, -STENCIL), n[2] + STENCIL - 1);
-        const uint32_t o = (uint32_t)dev_offset(g, cx, cy, cz);
-        bp[k] = pc[o] - pp[o];
-        bq[k] = qc[o] - qp[o];
-      }
+  for (int k = 0; k < kCPer; ++k) {
+    const int e = tid + k * kInferThreads;
+    cl[k] = -1;
+    cr[k] = make_float2(0.0f, 0.0f);
+    if (e < ncell) {
+      cl[k] = lidx[(size_t)t * ncell + e];
+      cr[k] = root[(size_t)t * ncell + e];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (bl[k] >= 0) {
+      sX[bl[k]] = bp[k];
+      sX[xQ + bl[k]] = bq[k];
     }
 #pragma unroll
-    for (int k = 0; k < kCPer; ++k) {
-      const int e = tid + k * kInferThreads;
-      cl[k] = -1;
-      cr[k] = make_float2(0.0f, 0.0f);
-      if (e < ncell) {
-        cl[k] = lidx[(size_t)t * ncell + e];
-        cr[k] = root[(size_t)t * ncell + e];
-      }
+  for (int k = 0; k < kCPer; ++k) {
+    const int e = tid + k * kInferThreads;
+    if (e < ncell) {
+      sL[e] = cl[k];
+      sX[xCx + e] = cr[k].x * g.cdt;
+      sX[xCz + e] = cr[k].y * g.cdt;
     }
-  };
-  const int t = blockIdx.x;
-  if (t >= ntiles) return;
-  {
-    // All of the tile's loads are issued before the first LDS write; the
-    // other workgroup on the CU computes meanwhile.
-    fetch(t);
-    {
-      const BoxGeom b = box_geom(tiles[t]);
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const int e = tid + k * kInferThreads;
-        if (e < b.nbox) {
-          const int bz = small_div(e, b.r01), r = e - bz * b.len0 * b.len1;
-          const int by = small_div(r, b.r0), bx = r - by * b.len0;
-          const int l = bx + by * b.ls1 + bz * b.ls2;
-          sX[l] = bp[k];
-          sX[xQ + l] = bq[k];
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kCPer; ++k) {
-      const int e = tid + k * kInferThreads;
-      if (e < ncell) {
-        sL[e] = cl[k];
-        sX[xCx + e] = cr[k].x * g.cdt;
-        sX[xCz + e] = cr[k].y * g.cdt;
-      }
-    }
-    __syncthreads();
-    const SlabTile T = tiles[t];
-    int lo[3], len[3];
-    slab_box(T, g, lo, len);
-    const int ls[3] = {1, len[0] | 1, (len[0] | 1) * len[1]};  // LDS strides
+  }
+  __syncthreads();
 
-    const int lane = tid & 63, wave = tid >> 6;
-    const int col = lane & 15, kb = lane >> 4;
-    const bool kb3 = kb == 3;
-    // Per-lane input offsets: slot 32s + 8kb + j.  Patch slots: relative to
-    // the cell's box index, in the tile's frame (outward normal, tangent 1,
-    // tangent 2).  Lanes kb = 3, j >= 3 read the one-hot table or the Courant
-    // numbers instead (below).
-    const int axis = T.face >> 1, side = T.face & 1;
-    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
-    const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
-    int off[2][8];  // bytes
+  const int lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, kb = lane >> 4;
+  const bool kb3 = kb == 3;
+  // Per-lane input offsets: slot 32s + 8kb + j.  Patch slots: relative to
+  // the cell's box index, in the tile's frame (outward normal, tangent 1,
+  // tangent 2).  Lanes kb = 3, j >= 3 read the one-hot table or the Courant
+  // numbers instead (below).
+  const int axis = T.face >> 1, side = T.face & 1;
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int sn = side ? ls[axis] : -ls[axis], s1 = ls[a1], s2 = ls[a2];
+  int off[2][8];  // bytes
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+  for (int s = 0; s < 2; ++s)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int dd = slot_desc(32 * s + 8 * kb + j);
-        off[s][j] = (dd & 1) ? 4 * ((s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
-                                    (((dd >> 3) & 3) - 1) * s1 +
-                                    (((dd >> 5) & 3) - 1) * s2)
-                             : 0;
+    for (int j = 0; j < 8; ++j) {
+      const int dd = slot_desc(32 * s + 8 * kb + j);
+      off[s][j] = (dd & 1) ? 4 * ((s ? xQ : 0) + (((dd >> 1) & 3) - 1) * sn +
+                                  (((dd >> 3) & 3) - 1) * s1 +
+                                  (((dd >> 5) & 3) - 1) * s2)
+                           : 0;
+    }
+  const char *const xb = reinterpret_cast<const char *>(sX);
+  // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
+  // lanes of cells outside the layer (their results are not stored).
+  const int lsafe = ls[0] + ls[1] + ls[2];
+  // A wave runs two groups of 16 cells at a time (rows 0-15 and 16-31 of a
+  // 32-cell slice of one depth level): every weight operand read from LDS
+  // feeds two MFMAs, and the two chains interleave.  Every wave takes the
+  // same number of slices (ncell is a multiple of 256).
+  constexpr int G = 2;
+  for (int g0 = wave * 16 * G; g0 < ncell; g0 += kInferThreads / 4 * G) {
+    // An offset the compiler cannot see is zero: it keeps the LDS operand
+    // reads inside the loop instead of holding all 18 in registers.
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the slice's cells, x 8
+    int e[G], l[G];
+    bool valid[G];
+    float m[G];
+    h8 X[G][2];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      e[q] = g0 + 16 * q + col;
+      const int lraw = sL[e[q]];
+      valid[q] = lraw >= 0;
+      l[q] = valid[q] ? lraw : lsafe;
+      const int lb = 4 * l[q];
+      float v[2][8];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int ad = lb + off[s][j];
+          if (j >= 3) {
+            const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
+                               : j < 6  ? xOne + d8 + (j + 2)
+                               : (j == 6 ? xCx : xCz) + e[q]);
+            ad = kb3 ? alt : ad;
+          }
+          v[s][j] = *reinterpret_cast<const float *>(xb + ad);
+        }
+      float mm = 0.0f;
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          mm = fmaxf(mm, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
+      mm = fmaxf(mm, __shfl_xor(mm, 16));
+      mm = fmaxf(mm, __shfl_xor(mm, 32));
+      m[q] = mm;
+      // One reciprocal per cell instead of 54 divisions; the inputs are
+      // rounded to half right after, which hides the last-bit difference.
+      const float rm = mm > 0.0f ? __builtin_amdgcn_rcpf(mm) : 0.0f;
+      const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          X[q][s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
+    }
+    h4 H1[G][4], H2[G][4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
+      f4 acc[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) acc[q] = bias;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const h8 w = *reinterpret_cast<const h8 *>(
+            &sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, X[q][s], acc[q], 0, 0, 0);
       }
-    const char *const xb = reinterpret_cast<const char *>(sX);
-    // A box cell whose 3x3x3 neighbourhood lies in the box: the stand-in for
-    // lanes of cells outside the layer (their results are not stored).
-    const int lsafe = ls[0] + ls[1] + ls[2];
-    // A wave runs two groups of 16 cells at a time (rows 0-15 and 16-31 of a
-    // 32-cell slice of one depth level): every weight operand read from LDS
-    // feeds two MFMAs, and the two chains interleave.  Every wave takes the
-    // same number of slices (ncell is a multiple of 256).
-    constexpr int G = 2;
-    for (int g0 = wave * 16 * G; g0 < ncell; g0 += kInferThreads / 4 * G) {
-      // An offset the compiler cannot see is zero: it keeps the LDS operand
-      // reads inside the loop instead of holding all 18 in registers.
-      int z = 0;
-      asm volatile("" : "+s"(z));
-      const int d8 = (g0 >> 8) * kMaxWidth;  // depth of the slice's cells, x 8
-      int e[G], l[G];
-      bool valid[G];
-      float m[G];
-      h8 X[G][2];
+#pragma unroll
+      for (int q = 0; q < G; ++q) H1[q][ot] = relu_pk(acc[q]);
+    }
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
+      f4 acc[G];
+#pragma unroll
+      for (int q = 0; q < G; ++q) acc[q] = bias;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const h8 w = *reinterpret_cast<const h8 *>(
+            &sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+              w, __builtin_shufflevector(H1[q][2 * s], H1[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+              acc[q], 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q) H2[q][ot] = relu_pk(acc[q]);
+    }
+    f4 y[G];
+    {
+      const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
+#pragma unroll
+      for (int q = 0; q < G; ++q) y[q] = bias;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const h8 w = *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          y[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+              w, __builtin_shufflevector(H2[q][2 * s], H2[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
+              y[q], 0, 0, 0);
+      }
+    }
+    if (lane < 16) {
 #pragma unroll
       for (int q = 0; q < G; ++q) {
-        e[q] = g0 + 16 * q + col;
-        const int lraw = sL[e[q]];
-        valid[q] = lraw >= 0;
-        l[q] = valid[q] ? lraw : lsafe;
-        const int lb = 4 * l[q];
-        float v[2][8];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            int ad = lb + off[s][j];
-            if (j >= 3) {
-              const int alt = 4 * (s == 0 ? xOne + d8 + (j - 3)
-                                 : j < 6  ? xOne + d8 + (j + 2)
-                                 : (j == 6 ? xCx : xCz) + e[q]);
-              ad = kb3 ? alt : ad;
-            }
-            v[s][j] = *reinterpret_cast<const float *>(xb + ad);
-          }
-        float mm = 0.0f;
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-          for (int j = 0; j < 8; ++j)
-            mm = fmaxf(mm, (j >= 3 && kb3) ? 0.0f : fabsf(v[s][j]));
-        mm = fmaxf(mm, __shfl_xor(mm, 16));
-        mm = fmaxf(mm, __shfl_xor(mm, 32));
-        m[q] = mm;
-        // One reciprocal per cell instead of 54 divisions; the inputs are
-        // rounded to half right after, which hides the last-bit difference.
-        const float rm = mm > 0.0f ? __builtin_amdgcn_rcpf(mm) : 0.0f;
-        const float rm3 = kb3 ? 1.0f : rm;  // slots j >= 3 of lanes kb = 3 as read
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-          for (int j = 0; j < 8; ++j)
-            X[q][s][j] = (_Float16)(v[s][j] * (j < 3 ? rm : rm3));
-      }
-      h4 H1[G][4], H2[G][4];
-#pragma unroll
-      for (int ot = 0; ot < 4; ++ot) {
-        const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 16 * ot + 4 * kb]);
-        f4 acc[G];
-#pragma unroll
-        for (int q = 0; q < G; ++q) acc[q] = bias;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const h8 w = *reinterpret_cast<const h8 *>(
-              &sW[z + lW1 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
-#pragma unroll
-          for (int q = 0; q < G; ++q)
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, X[q][s], acc[q], 0, 0, 0);
+        if (!valid[q]) continue;
+        float2 out;
+        if (mode == kOutDamping) {
+          const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][0]));
+          const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][1]));
+          out = make_float2(-sp * sX[l[q]], -sq * sX[xQ + l[q]]);
+        } else {
+          out = make_float2(m[q] * y[q][0], m[q] * y[q][1]);
         }
-#pragma unroll
-        for (int q = 0; q < G; ++q) H1[q][ot] = relu_pk(acc[q]);
-      }
-#pragma unroll
-      for (int ot = 0; ot < 4; ++ot) {
-        const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + kHid + 16 * ot + 4 * kb]);
-        f4 acc[G];
-#pragma unroll
-        for (int q = 0; q < G; ++q) acc[q] = bias;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const h8 w = *reinterpret_cast<const h8 *>(
-              &sW[z + lW2 + (16 * ot + col) * kRow + 32 * s + 8 * kb]);
-#pragma unroll
-          for (int q = 0; q < G; ++q)
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                w, __builtin_shufflevector(H1[q][2 * s], H1[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
-                acc[q], 0, 0, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < G; ++q) H2[q][ot] = relu_pk(acc[q]);
-      }
-      f4 y[G];
-      {
-        const f4 bias = *reinterpret_cast<const f4 *>(&sB[z + 2 * kHid + 4 * kb]);
-#pragma unroll
-        for (int q = 0; q < G; ++q) y[q] = bias;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const h8 w = *reinterpret_cast<const h8 *>(&sW[z + lW3 + col * kRow + 32 * s + 8 * kb]);
-#pragma unroll
-          for (int q = 0; q < G; ++q)
-            y[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                w, __builtin_shufflevector(H2[q][2 * s], H2[q][2 * s + 1], 0, 1, 2, 3, 4, 5, 6, 7),
-                y[q], 0, 0, 0);
-        }
-      }
-      if (lane < 16) {
-#pragma unroll
-        for (int q = 0; q < G; ++q) {
-          if (!valid[q]) continue;
-          float2 out;
-          if (mode == kOutDamping) {
-            const float sp = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][0]));
-            const float sq = sigma_max * __builtin_amdgcn_rcpf(1.0f + __expf(-y[q][1]));
-            out = make_float2(-sp * sX[l[q]], -sq * sX[xQ + l[q]]);
-          } else {
-            out = make_float2(m[q] * y[q][0], m[q] * y[q][1]);
-          }
-          delta[(size_t)t * ncell + e[q]] = out;
-        }
+        delta[(size_t)t * ncell + e[q]] = out;
       }
     }
   }
