@@ -427,7 +427,11 @@ __global__ void __launch_bounds__(kInferThreads)
   if (tid < kMaxWidth * kMaxWidth)
     sX[xOne + tid] = (tid >> 3) == (tid & 7) ? 1.0f : 0.0f;
 
-  const int t = blockIdx.x;
+  // Workgroups are dealt to the 8 XCDs round-robin; giving each XCD a
+  // contiguous run of tiles keeps the box lines that neighbouring tiles share
+  // in one L2.
+  int t = blockIdx.x;
+  if (ntiles % 8 == 0) t = (t % 8) * (ntiles / 8) + t / 8;
   if (t >= ntiles) return;
   const SlabTile T = tiles[t];
   const int ncell = kTile * kTile * g.width;
