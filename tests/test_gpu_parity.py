@@ -145,6 +145,25 @@ def test_prefetch_depths_are_bitwise_equal(H, VM):
             assert np.array_equal(bits(a), bits(b)), k
 
 
+def test_default_z_chunk_is_bitwise_equal_to_whole_columns(H, VM):
+    """The default z chunk is chosen per launch (two tiles per CU): the
+    192 x 192 x 112 compute cells of these arrays make 72 columns of 32 x 16
+    cells, so the launch splits them into 32-plane chunks.  Every cell's
+    arithmetic is the same in any chunk: the fields match whole 512-plane
+    columns bit for bit."""
+    w = VM.random_small((120, 200, 200), seed=14, steps=12)
+    amp = w.source_table()
+    out = []
+    for zc in (0, 512):
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        assert d.set_tiling(32, 16, zc)
+        d.run(12, w.src_index, amp, graph=True)
+        out.append(d.fields())
+        d.close()
+    for a, b in zip(*out):
+        assert np.array_equal(bits(a), bits(b))
+
+
 def test_stream_probe_touches_the_compute_region_only(H, VM):
     """The stencil-free probe (one untimed pass + reps timed passes) writes
     pp <- 2 pc - pp + dt * sum(coef) with dt = 0 at every compute cell and
