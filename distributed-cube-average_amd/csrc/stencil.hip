@@ -510,8 +510,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
             F::st(Gqp, bo, dnq);
           }
         } else {
-          float pn = 2.0f * rp[J] - opp + rhsp * dt2;
-          float qn = 2.0f * rq[J] - oqp + rhsq * dt2;
+          // (rhs * dt) * dt, the reference's association
+          // (include/sample_compute.h:181-184).
+          float pn = 2.0f * rp[J] - opp + rhsp * a.dt * a.dt;
+          float qn = 2.0f * rq[J] - oqp + rhsq * a.dt * a.dt;
           if (src) {
             pn += src_amp;
             qn += src_amp;
