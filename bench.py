@@ -453,9 +453,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build runs one rank's "
-              f"subdomain per GPU and no multi-GPU path is part of it "
-              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        print(f"bench.py measures one GPU: the library steps one rank's "
+              f"subdomain per GPU, and the exchange between GPUs is not part "
+              f"of it (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     if args.child_pmc:
         pmc_child(args)
