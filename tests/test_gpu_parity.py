@@ -127,6 +127,53 @@ def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
                           want[inside] - old[inside]) < FAST_TOL
 
 
+def test_empty_boxes_are_no_ops(H, VM):
+    """A box with end <= start on any axis updates nothing, as the
+    reference's loops over [start, end) do nothing (src/openmp_propagate.c)."""
+    shape = (20, 22, 24)
+    w = VM.random_small(shape, seed=15)
+    rng = np.random.default_rng(16)
+    f0 = [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+    for kernel in (H.KERNEL_EXACT, H.KERNEL_FAST):
+        d = H.DeviceData(w.coef, f0, w.dx, w.dy, w.dz, w.dt, kernel=kernel)
+        for box in [((4, 4, 4), (4, 18, 16)), ((4, 4, 4), (20, 4, 16)),
+                    ((4, 4, 9), (20, 18, 9)), ((10, 10, 10), (5, 5, 5))]:
+            d.propagate(*box)
+        got = d.fields()
+        d.close()
+        for a, b in zip(got, f0):
+            assert np.array_equal(bits(a), bits(b)), kernel
+
+
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_smallest_grids(H, O, VM, kernel):
+    """9 x 10 x 11 arrays (compute regions of 1 x 2 x 3 cells, far smaller
+    than one tile): the exact kernel bit-identical to the oracle, the fast
+    kernel within the bar."""
+    shape = (11, 10, 9)  # (sz, sy, sx)
+    w = VM.random_small(shape, seed=17)
+    rng = np.random.default_rng(18)
+    f0 = [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+    d = H.DeviceData(w.coef, f0, w.dx, w.dy, w.dz, w.dt, kernel=kernel)
+    box = ((4, 4, 4), (5, 6, 7))
+    d.propagate(*box)
+    pp, pc, qp, qc = d.fields()
+    d.close()
+    ref = [a.copy() for a in f0]
+    O.propagate_f32(box[0], box[1], ref[0], ref[1], ref[2], ref[3], w.coef,
+                    w.dx, w.dy, w.dz, w.dt)
+    if kernel == H.KERNEL_EXACT:
+        assert np.array_equal(bits(pp), bits(ref[0]))
+        assert np.array_equal(bits(qp), bits(ref[2]))
+    else:
+        inside = np.zeros(shape, bool)
+        inside[4:7, 4:6, 4:5] = True
+        for got, want, old in ((pp, ref[0], f0[0]), (qp, ref[2], f0[2])):
+            assert np.array_equal(bits(got[~inside]), bits(old[~inside]))
+            assert rel_l2(got[inside] - old[inside],
+                          want[inside] - old[inside]) < FAST_TOL
+
+
 def test_prefetch_depths_are_bitwise_equal(H, VM):
     """The prefetch depth changes only when loads are issued: 1, 2 and 3
     give the same bits over a fused 30-step loop with a source."""
