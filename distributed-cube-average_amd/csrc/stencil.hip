@@ -248,7 +248,7 @@ struct StepLoads {
 };
 // Own and apron values of a staged plane.
 struct Staged {
-  float cp, cq, ap, aq;
+  float cp, cq, ap, aq, ap2, aq2;
 };
 
 // PF = 1: the staged plane is fetched one plane ahead and the step's
@@ -269,7 +269,9 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   constexpr int PL = PW * PH;      // cells of one staged plane
   constexpr int NAPR = PL - TX * TY;
   constexpr int NDXA = 8 * TX;     // D1x values of the 8 apron rows
-  static_assert(NAPR <= NT, "apron must fit one cell per thread");
+  // Apron cells per thread: one for tiles of 16+ rows, two for 8-row tiles.
+  constexpr bool kApron2 = NAPR > NT;
+  static_assert(NAPR <= 2 * NT, "apron must fit two cells per thread");
   static_assert(NDXA <= NT, "D1x apron must fit one cell per thread");
 
   // One buffer each suffices: every LDS read of plane n happens before the
@@ -299,30 +301,37 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   const int lo = (ty + 4) * PW + tx + 4;  // own cell in the LDS plane
   const int ld = (ty + 4) * TX + tx;      // own cell in the D1x plane
 
-  // Apron cell of this thread (top rows, bottom rows, then side columns).
-  int la = -1, oa = 0;
-  if (tid < NAPR) {
+  // Apron cell t of the staged plane (top rows, bottom rows, then side
+  // columns): its LDS index (-1 past the apron) and in-plane array offset.
+  auto apron_cell = [&](int t, int &la_, int &oa_) {
+    la_ = -1;
+    oa_ = 0;
+    if (t >= NAPR) return;
     int ax, ay;
-    if (tid < 4 * PW) {
-      ay = tid / PW;
-      ax = tid % PW;
-    } else if (tid < 8 * PW) {
-      const int t = tid - 4 * PW;
-      ay = TY + 4 + t / PW;
+    if (t < 4 * PW) {
+      ay = t / PW;
       ax = t % PW;
+    } else if (t < 8 * PW) {
+      const int s = t - 4 * PW;
+      ay = TY + 4 + s / PW;
+      ax = s % PW;
     } else {
-      const int t = tid - 8 * PW;
-      ay = 4 + t / 8;
-      const int c8 = t % 8;
+      const int s = t - 8 * PW;
+      ay = 4 + s / 8;
+      const int c8 = s % 8;
       ax = c8 < 4 ? c8 : TX + c8;
     }
-    la = ay * PW + ax;
+    la_ = ay * PW + ax;
     int gx = tx0 + ax - 4, gy = ty0 + ay - 4;
     gx = gx < xl ? gx : xl;
     gy = gy < yl ? gy : yl;
-    oa = gx + gy * ys;
-  }
+    oa_ = gx + gy * ys;
+  };
+  int la, oa, la2 = -1, oa2 = 0;
+  apron_cell(tid, la, oa);
+  if (kApron2) apron_cell(tid + NT, la2, oa2);
   const uint32_t oab = (uint32_t)oa * F::kBytes;
+  const uint32_t oab2 = (uint32_t)oa2 * F::kBytes;
   // D1x of the apron rows: rows 0..3 and TY+4..TY+7 of the D1x plane.
   int dr = -1, dc = 0;
   if (tid < NDXA) {
@@ -355,13 +364,17 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 
   const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
   auto load_staged = [&](int k) {
-    Staged v{0.f, 0.f, 0.f, 0.f};
+    Staged v{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const uint32_t bb = (uint32_t)(k < nend ? k : nend - 1) * zsb;
     v.cp = F::ld(Gpc, bb + ob);
     v.cq = F::ld(Gqc, bb + ob);
     if (la >= 0) {
       v.ap = F::ld(Gpc, bb + oab);
       v.aq = F::ld(Gqc, bb + oab);
+    }
+    if (kApron2 && la2 >= 0) {
+      v.ap2 = F::ld(Gpc, bb + oab2);
+      v.aq2 = F::ld(Gqc, bb + oab2);
     }
     return v;
   };
@@ -409,6 +422,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       if (la >= 0) {
         sP[la] = cur.ap;
         sQ[la] = cur.aq;
+      }
+      if (kApron2 && la2 >= 0) {
+        sP[la2] = cur.ap2;
+        sQ[la2] = cur.aq2;
       }
       __syncthreads();
 
@@ -579,8 +596,7 @@ __global__ void __launch_bounds__(TX *TY)
 }
 
 bool fast_tiling_supported(int tx, int ty) {
-  // The apron (one plane's 4-cell frame) must fit one cell per thread.
-  return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && ty == 16);
+  return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && (ty == 8 || ty == 16));
 }
 
 template <int TX, int TY, int FS, int CS>
@@ -691,6 +707,7 @@ void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
   if (tile_x == 32 && tile_y == 16) launch_fast_tile<32, 16>(b, grid, block, s);
   else if (tile_x == 32 && tile_y == 32) launch_fast_tile<32, 32>(b, grid, block, s);
   else if (tile_x == 64 && tile_y == 16) launch_fast_tile<64, 16>(b, grid, block, s);
+  else if (tile_x == 64 && tile_y == 8) launch_fast_tile<64, 8>(b, grid, block, s);
   else DCX_FAIL("unsupported tile %dx%d", tile_x, tile_y);
   DCX_CHECK(hipGetLastError());
 }

@@ -37,7 +37,7 @@ enum { DCX_KERNEL_FAST = 0, DCX_KERNEL_EXACT = 1 };
 int dcx_set_kernel(dc_device_data *data, int kernel);
 
 /* Tile shape of the fast kernel: (tile_x, tile_y) one of (32,16), (32,32),
- * (64,16); z_chunk >= 1 planes per workgroup, or 0 (the default) to let each
+ * (64,16), (64,8); z_chunk >= 1 planes per workgroup, or 0 (the default) to let each
  * launch pick the longest chunk, from 32 to 512 planes, that gives two tiles
  * per compute unit.  Returns 0, or -1 (and changes nothing) for an
  * unsupported shape. */

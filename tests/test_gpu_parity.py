@@ -99,7 +99,8 @@ def test_forty_step_loop(H, O, kernel, mode):
 @pytest.mark.parametrize("tiling", [(32, 16, 64, 1), (32, 16, 5, 1),
                                     (32, 16, 64, 2), (32, 16, 3, 2),
                                     (32, 16, 64, 3), (32, 16, 2, 3),
-                                    (32, 32, 7, 1), (64, 16, 64, 2)])
+                                    (32, 32, 7, 1), (64, 16, 64, 2),
+                                    (64, 8, 64, 2), (64, 8, 5, 1)])
 def test_fast_kernel_ragged_grid_and_boxes(H, O, VM, tiling):
     """Odd sizes (tile overhang in x, y and z) and sub-boxes of the grid."""
     shape = (29, 41, 70)  # (sz, sy, sx)
