@@ -179,10 +179,19 @@ __device__ __forceinline__ void stb(void *base, uint32_t boff, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
 }
 
-// Field storage policies (DCX_STORE_*): element size, load as float, store.
+// Field storage policies (DCX_STORE_*): element size, raw load, decode to
+// float, store.  Prefetched values stay raw until they are used: a decode
+// placed at the load would make the wave wait for the load there, before the
+// barrier that ends the iteration, and undo the prefetch.
 template <int FS> struct FieldIO;
 template <> struct FieldIO<DCX_STORE_F32> {
   static constexpr uint32_t kBytes = 4;
+  static __device__ __forceinline__ uint32_t ldr(const void *b, uint32_t o) {
+    return ldb<uint32_t>(b, o);
+  }
+  static __device__ __forceinline__ float dec(uint32_t r) {
+    return __uint_as_float(r);
+  }
   static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
     return ldb<float>(b, o);
   }
@@ -192,8 +201,14 @@ template <> struct FieldIO<DCX_STORE_F32> {
 };
 template <> struct FieldIO<DCX_STORE_F16> {
   static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ uint32_t ldr(const void *b, uint32_t o) {
+    return ldb<uint16_t>(b, o);
+  }
+  static __device__ __forceinline__ float dec(uint32_t r) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)r);
+  }
   static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
-    return (float)__builtin_bit_cast(_Float16, ldb<uint16_t>(b, o));
+    return dec(ldr(b, o));
   }
   static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
     stb<uint16_t>(b, o, __builtin_bit_cast(uint16_t, (_Float16)v));
@@ -201,8 +216,14 @@ template <> struct FieldIO<DCX_STORE_F16> {
 };
 template <> struct FieldIO<DCX_STORE_BF16> {
   static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ uint32_t ldr(const void *b, uint32_t o) {
+    return ldb<uint16_t>(b, o);
+  }
+  static __device__ __forceinline__ float dec(uint32_t r) {
+    return __uint_as_float(r << 16);
+  }
   static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
-    return __uint_as_float((uint32_t)ldb<uint16_t>(b, o) << 16);
+    return dec(ldr(b, o));
   }
   static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
     uint32_t u = __float_as_uint(v);
@@ -211,44 +232,55 @@ template <> struct FieldIO<DCX_STORE_BF16> {
   }
 };
 
-// Coefficient storage policies (DCX_COEF_*): element size and decode of
-// array k (a compile-time index after unrolling).
+// Coefficient storage policies (DCX_COEF_*): element size, raw load of
+// array k (a compile-time index after unrolling) and its decode, kept apart
+// for the reason given at FieldIO.
 template <int CS> struct CoefIO;
 template <> struct CoefIO<DCX_COEF_F32> {
   static constexpr uint32_t kBytes = 4;
-  static __device__ __forceinline__ float ld(const StepArgs &a, int k,
-                                             uint32_t o) {
-    return ldb<float>(a.cf[k], o);
+  static __device__ __forceinline__ uint32_t ldr(const StepArgs &a, int k,
+                                                 uint32_t o) {
+    return ldb<uint32_t>(a.cf[k], o);
+  }
+  static __device__ __forceinline__ float dec(const StepArgs &, int,
+                                              uint32_t r) {
+    return __uint_as_float(r);
   }
 };
 template <> struct CoefIO<DCX_COEF_F16> {
   static constexpr uint32_t kBytes = 2;
-  static __device__ __forceinline__ float ld(const StepArgs &a, int k,
-                                             uint32_t o) {
-    return (float)__builtin_bit_cast(_Float16, ldb<uint16_t>(a.cf[k], o)) *
-           a.c_st[k];
+  static __device__ __forceinline__ uint32_t ldr(const StepArgs &a, int k,
+                                                 uint32_t o) {
+    return ldb<uint16_t>(a.cf[k], o);
+  }
+  static __device__ __forceinline__ float dec(const StepArgs &a, int k,
+                                              uint32_t r) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)r) * a.c_st[k];
   }
 };
 template <> struct CoefIO<DCX_COEF_U16> {
   static constexpr uint32_t kBytes = 2;
-  static __device__ __forceinline__ float ld(const StepArgs &a, int k,
-                                             uint32_t o) {
-    const uint32_t u = ldb<uint16_t>(a.cf[k], o);
+  static __device__ __forceinline__ uint32_t ldr(const StepArgs &a, int k,
+                                                 uint32_t o) {
+    return ldb<uint16_t>(a.cf[k], o);
+  }
+  static __device__ __forceinline__ float dec(const StepArgs &a, int k,
+                                              uint32_t u) {
     const float t = fmaf((float)u, a.c_st[k], a.c_lo[k]);
     if (k < V2PX) return t;              // angle terms: affine
     return u ? exp2f(t) : 0.0f;          // v^2 terms: affine in log2
   }
 };
 
-// Per-step global inputs: coefficients of the staged plane (in-plane part of
-// H1) and of the output plane, and the output plane's old state (previous
-// level for leapfrog storage, time difference otherwise).
+// Per-step global inputs, raw: coefficients of the staged plane (in-plane
+// part of H1) and of the output plane, and the output plane's old state
+// (previous level for leapfrog storage, time difference otherwise).
 struct StepLoads {
-  float cxx, cyy, cxy, czz, cxz, cyz, vpx, vpz, vsz, vpn, opp, oqp;
+  uint32_t cxx, cyy, cxy, czz, cxz, cyz, vpx, vpz, vsz, vpn, opp, oqp;
 };
-// Own and apron values of a staged plane.
+// Own and apron values of a staged plane, raw.
 struct Staged {
-  float cp, cq, ap, aq, ap2, aq2;
+  uint32_t cp, cq, ap, aq, ap2, aq2;
 };
 
 // PF = 1: the staged plane is fetched one plane ahead and the step's
@@ -364,17 +396,17 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 
   const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
   auto load_staged = [&](int k) {
-    Staged v{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    Staged v{0u, 0u, 0u, 0u, 0u, 0u};
     const uint32_t bb = (uint32_t)(k < nend ? k : nend - 1) * zsb;
-    v.cp = F::ld(Gpc, bb + ob);
-    v.cq = F::ld(Gqc, bb + ob);
+    v.cp = F::ldr(Gpc, bb + ob);
+    v.cq = F::ldr(Gqc, bb + ob);
     if (la >= 0) {
-      v.ap = F::ld(Gpc, bb + oab);
-      v.aq = F::ld(Gqc, bb + oab);
+      v.ap = F::ldr(Gpc, bb + oab);
+      v.aq = F::ldr(Gqc, bb + oab);
     }
     if (kApron2 && la2 >= 0) {
-      v.ap2 = F::ld(Gpc, bb + oab2);
-      v.aq2 = F::ld(Gqc, bb + oab2);
+      v.ap2 = F::ldr(Gpc, bb + oab2);
+      v.aq2 = F::ldr(Gqc, bb + oab2);
     }
     return v;
   };
@@ -387,18 +419,18 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
     const uint32_t bc = (uint32_t)kc * zsbc + obc;
     const uint32_t bjc = (uint32_t)kj * zsbc + obc;
     const uint32_t bjf = (uint32_t)kj * zsb + ob;
-    v.cxx = Cf::ld(a, CXX, bc);
-    v.cyy = Cf::ld(a, CYY, bc);
-    v.cxy = Cf::ld(a, CXY, bc);
-    v.czz = Cf::ld(a, CZZ, bjc);
-    v.cxz = Cf::ld(a, CXZ, bjc);
-    v.cyz = Cf::ld(a, CYZ, bjc);
-    v.vpx = Cf::ld(a, V2PX, bjc);
-    v.vpz = Cf::ld(a, V2PZ, bjc);
-    v.vsz = Cf::ld(a, V2SZ, bjc);
-    v.vpn = Cf::ld(a, V2PN, bjc);
-    v.opp = F::ld(Gpp, bjf);
-    v.oqp = F::ld(Gqp, bjf);
+    v.cxx = Cf::ldr(a, CXX, bc);
+    v.cyy = Cf::ldr(a, CYY, bc);
+    v.cxy = Cf::ldr(a, CXY, bc);
+    v.czz = Cf::ldr(a, CZZ, bjc);
+    v.cxz = Cf::ldr(a, CXZ, bjc);
+    v.cyz = Cf::ldr(a, CYZ, bjc);
+    v.vpx = Cf::ldr(a, V2PX, bjc);
+    v.vpz = Cf::ldr(a, V2PZ, bjc);
+    v.vsz = Cf::ldr(a, V2SZ, bjc);
+    v.vpn = Cf::ldr(a, V2PN, bjc);
+    v.opp = F::ldr(Gpp, bjf);
+    v.oqp = F::ldr(Gqp, bjf);
     return v;
   };
 
@@ -417,15 +449,16 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       const int n = nb + u;
       if (n >= nend) break;
       // Stage plane n.
-      sP[lo] = cur.cp;
-      sQ[lo] = cur.cq;
+      const float cp = F::dec(cur.cp), cq = F::dec(cur.cq);
+      sP[lo] = cp;
+      sQ[lo] = cq;
       if (la >= 0) {
-        sP[la] = cur.ap;
-        sQ[la] = cur.aq;
+        sP[la] = F::dec(cur.ap);
+        sQ[la] = F::dec(cur.aq);
       }
       if (kApron2 && la2 >= 0) {
-        sP[la2] = cur.ap2;
-        sQ[la2] = cur.aq2;
+        sP[la2] = F::dec(cur.ap2);
+        sQ[la2] = F::dec(cur.aq2);
       }
       __syncthreads();
 
@@ -446,10 +479,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         L = load_step(n);
         nxt = load_staged(n + 1);
       }
-      const float cp = cur.cp, cq = cur.cq;
-      const float cxx = L.cxx, cyy = L.cyy, cxy = L.cxy, czz = L.czz,
-                  cxz = L.cxz, cyz = L.cyz, vpx = L.vpx, vpz = L.vpz,
-                  vsz = L.vsz, vpn = L.vpn, opp = L.opp, oqp = L.oqp;
+      // L stays raw here; each value is decoded where it is used (the
+      // output block, the ring push), after this iteration's barriers.
 
       // In-plane derivatives of plane n.
       const Pairs pxq = lds_pairs(sP, lo, 1), pyq = lds_pairs(sP, lo, PW);
@@ -499,6 +530,11 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
                            W_L3 * (RS(yq, 3) - RS(yq, 5)) +
                            W_L4 * (dyq - yq[O])) * a.dyz;
 #undef RS
+        const float czz = Cf::dec(a, CZZ, L.czz), cxz = Cf::dec(a, CXZ, L.cxz),
+                    cyz = Cf::dec(a, CYZ, L.cyz), vpx = Cf::dec(a, V2PX, L.vpx),
+                    vpz = Cf::dec(a, V2PZ, L.vpz), vsz = Cf::dec(a, V2SZ, L.vsz),
+                    vpn = Cf::dec(a, V2PN, L.vpn);
+        const float opp = F::dec(L.opp), oqp = F::dec(L.oqp);
         const int D = u & 3;
         const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
         const float h2p = Sp[D] + pzz - h1p;
@@ -548,6 +584,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       yp[u] = dyp;
       xq[u] = dxq;
       yq[u] = dyq;
+      const float cxx = Cf::dec(a, CXX, L.cxx), cyy = Cf::dec(a, CYY, L.cyy),
+                  cxy = Cf::dec(a, CXY, L.cxy);
       Sp[u & 3] = pxx + pyy;
       Hp[u & 3] = cxx * pxx + cyy * pyy + cxy * pxy;
       Sq[u & 3] = qxx + qyy;
