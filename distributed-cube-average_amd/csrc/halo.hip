@@ -13,8 +13,38 @@ namespace dcx {
 
 // The packed stream of all boxes is one index range; each lane finds the box
 // of its element by binary search over the box offsets held in LDS.  Box
-// volumes are below 2^32, so in-box coordinates use 32-bit division.
+// volumes are below 2^32, so in-box coordinates are 32-bit.
 constexpr int kMaxBoxesLds = 512;
+
+// Unsigned 32-bit division by a divisor fixed per box, as a multiply-high and
+// two shifts (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, fig. 4.1): exact for every 32-bit numerator and
+// every divisor 1 <= d <= 2^31.
+struct InvDiv {
+  uint32_t m;
+  uint32_t s1, s2;
+};
+
+__device__ __forceinline__ InvDiv inv_div(uint32_t d) {
+  const uint32_t l = d > 1 ? 32u - (uint32_t)__clz(d - 1) : 0u;  // ceil(log2 d)
+  const uint64_t m = ((uint64_t)1 << 32) * (((uint64_t)1 << l) - d) / d + 1;
+  return {(uint32_t)m, l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
+}
+
+__device__ __forceinline__ uint32_t div_by(uint32_t n, const InvDiv &v) {
+  const uint32_t t = __umulhi(v.m, n);
+  return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
+// What a lane needs of its box, staged in LDS by every block.
+struct BoxRec {
+  float *arr;
+  int64_t base;  // array index of the box's first element
+  int64_t ys, zs;
+  uint32_t nx, ny;
+  InvDiv dx, dy;
+  int esize;
+};
 
 // Element i of the box's array <-> element e of the packed buffer, in the
 // box's element size (the buffer uses the same element size).
@@ -35,11 +65,33 @@ __device__ __forceinline__ void copy_elem(const BoxCopy &b, int64_t i,
   }
 }
 
+// Dynamic LDS: nbox + 1 offsets, then nbox records.
+static size_t box_copy_lds(int nbox) {
+  return (size_t)(nbox + 1) * sizeof(int64_t) + (size_t)nbox * sizeof(BoxRec);
+}
+
 __global__ void __launch_bounds__(256)
     k_box_copy(const BoxCopy *__restrict__ boxes, int nbox, int64_t total,
                float *__restrict__ buf, int dir) {
-  __shared__ int64_t soff[kMaxBoxesLds + 1];
-  for (int k = threadIdx.x; k < nbox; k += blockDim.x) soff[k] = boxes[k].off;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  int64_t *soff = reinterpret_cast<int64_t *>(lds_raw);
+  BoxRec *srec = reinterpret_cast<BoxRec *>(lds_raw + (size_t)(nbox + 1) *
+                                                          sizeof(int64_t));
+  for (int k = threadIdx.x; k < nbox; k += blockDim.x) {
+    const BoxCopy &b = boxes[k];
+    soff[k] = b.off;
+    BoxRec r;
+    r.arr = b.arr;
+    r.base = b.x0 + b.y0 * b.ys + b.z0 * b.zs;
+    r.ys = b.ys;
+    r.zs = b.zs;
+    r.nx = (uint32_t)b.nx;
+    r.ny = (uint32_t)b.ny;
+    r.dx = inv_div(r.nx > 0 ? r.nx : 1);
+    r.dy = inv_div(r.ny > 0 ? r.ny : 1);
+    r.esize = b.esize;
+    srec[k] = r;
+  }
   if (threadIdx.x == 0) soff[nbox] = total;
   __syncthreads();
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -52,15 +104,26 @@ __global__ void __launch_bounds__(256)
       else
         hi = mid - 1;
     }
-    const BoxCopy &b = boxes[lo];
-    const uint32_t k = (uint32_t)(e - b.off);
-    const uint32_t nx = (uint32_t)b.nx, ny = (uint32_t)b.ny;
-    const uint32_t r = k / nx;
-    const int64_t x = k - r * nx;
-    const int64_t z = r / ny;
-    const int64_t y = r - (uint32_t)z * ny;
-    const int64_t i = (b.x0 + x) + (b.y0 + y) * b.ys + (b.z0 + z) * b.zs;
-    copy_elem(b, i, buf, e, dir);
+    const BoxRec &b = srec[lo];
+    const uint32_t k = (uint32_t)(e - soff[lo]);
+    const uint32_t r = div_by(k, b.dx);
+    const uint32_t x = k - r * b.nx;
+    const uint32_t z = div_by(r, b.dy);
+    const uint32_t y = r - z * b.ny;
+    const int64_t i = b.base + x + (int64_t)y * b.ys + (int64_t)z * b.zs;
+    if (b.esize == 2) {
+      uint16_t *arr = reinterpret_cast<uint16_t *>(b.arr);
+      uint16_t *pk = reinterpret_cast<uint16_t *>(buf);
+      if (dir == 0)
+        pk[e] = arr[i];
+      else
+        arr[i] = pk[e];
+    } else {
+      if (dir == 0)
+        buf[e] = b.arr[i];
+      else
+        b.arr[i] = buf[e];
+    }
   }
 }
 
@@ -90,8 +153,8 @@ void launch_box_copy(const BoxCopy *boxes, int nbox, int64_t total, float *buf,
                      int dir, hipStream_t s) {
   if (nbox <= 0 || total <= 0) return;
   if (nbox > kMaxBoxesLds) DCX_FAIL("too many boxes in one copy: %d", nbox);
-  hipLaunchKernelGGL(k_box_copy, dim3(copy_blocks(total)), dim3(256), 0, s,
-                     boxes, nbox, total, buf, dir);
+  hipLaunchKernelGGL(k_box_copy, dim3(copy_blocks(total)), dim3(256),
+                     box_copy_lds(nbox), s, boxes, nbox, total, buf, dir);
   DCX_CHECK(hipGetLastError());
 }
 

@@ -363,6 +363,29 @@ class DeviceData:
         ptrs = [a.ctypes.data_as(C.POINTER(C.c_size_t)) for a in arr]
         load().dcx_set_halo_loopback(self.ptr, n, *ptrs)
 
+    def pack_unpack(self, send_boxes, src: str, recv_boxes, dst: str):
+        """dcx_pack_boxes: send_boxes of field src into the struct's device
+        scratch buffer (one launch); then dcx_unpack_boxes: that buffer into
+        recv_boxes of field dst (one launch).  Boxes as
+        ((x0,y0,z0),(x1,y1,z1)); both lists must hold the same number of
+        cells."""
+        def vol(boxes):
+            return sum(int(np.prod([e - s for s, e in zip(*b)])) for b in boxes)
+
+        n = vol(send_boxes)
+        if n != vol(recv_boxes):
+            raise ValueError(f"send boxes hold {n} cells, receive boxes "
+                             f"{vol(recv_boxes)}")
+        arr = [np.ascontiguousarray(np.array([b[i] for b in boxes], np.uint64)
+                                    .reshape(-1)) for boxes in
+               (send_boxes, recv_boxes) for i in (0, 1)]
+        ptrs = [a.ctypes.data_as(C.POINTER(C.c_size_t)) for a in arr]
+        buf = load().dcx_scratch(self.ptr, max(n, 1))
+        load().dcx_pack_boxes(self.ptr, buf, len(send_boxes), ptrs[0], ptrs[1],
+                              self.field_token(src))
+        load().dcx_unpack_boxes(self.ptr, buf, len(recv_boxes), ptrs[2],
+                                ptrs[3], self.field_token(dst))
+
     def stream_probe(self, reps: int = 20) -> float:
         """ms per pointwise pass with the step's bytes and no stencil
         (overwrites pp and qp)."""

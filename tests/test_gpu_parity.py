@@ -281,6 +281,59 @@ def test_halo_extract_insert_bitwise(H, O, VM):
     d.close()
 
 
+def test_multi_box_pack_unpack_bitwise(H, VM):
+    """Many boxes per launch (dcx_pack_boxes / dcx_unpack_boxes): faces one
+    to four cells thin, an edge, a corner, one cell, an empty box, a ragged
+    interior box and the array's last row.  The packed stream is unpacked as
+    consecutive x-rows of another field, so that field's leading cells hold
+    the stream itself: box after box, each x fastest, then y, then z (the
+    order of src/device_data.c:61-92).  Then the inverse: rows packed and
+    unpacked into the boxes.  Hundreds of row boxes also exercise the box
+    search of one launch."""
+    shape = (23, 37, 45)  # (sz, sy, sx)
+    sz, sy, sx = shape
+    w = VM.random_small(shape, seed=9)
+    rng = np.random.default_rng(4)
+    f0 = [rng.standard_normal(shape).astype(np.float32) for _ in range(4)]
+    boxes = [((4, 4, 4), (8, 33, 19)), ((8, 4, 4), (41, 8, 19)),
+             ((8, 8, 15), (41, 33, 19)), ((0, 0, 0), (4, 4, 23)),
+             ((41, 33, 19), (45, 37, 23)), ((2, 30, 20), (3, 31, 21)),
+             ((5, 5, 5), (5, 9, 9)), ((8, 8, 2), (40, 29, 12)),
+             ((0, 36, 22), (41, 37, 23)), ((44, 0, 0), (45, 33, 19)),
+             ((4, 3, 0), (43, 4, 23))]
+    # Disjoint, as halo boxes are: one launch writes its boxes in no set
+    # order, so overlapping destinations would have no defined result.
+    cover = np.zeros(shape, int)
+    for (x0, y0, z0), (x1, y1, z1) in boxes:
+        cover[z0:z1, y0:y1, x0:x1] += 1
+    assert cover.max() == 1
+    stream = np.concatenate([f0[1][z0:z1, y0:y1, x0:x1].ravel()
+                             for (x0, y0, z0), (x1, y1, z1) in boxes])
+    total = stream.size
+    rows = []
+    for t in range(-(-total // sx)):
+        z, y = divmod(t, sy)
+        rows.append(((0, y, z), (min(sx, total - t * sx), y + 1, z + 1)))
+    assert 256 < len(rows) <= 512
+    say("multi-box", len(boxes), "boxes,", len(rows), "rows,", total, "cells")
+    d = H.DeviceData(w.coef, f0)
+    d.pack_unpack(boxes, "pc", rows, "qp")
+    want = f0[2].copy()
+    want.reshape(-1)[:total] = stream
+    assert np.array_equal(bits(d.download(2)), bits(want))
+    d.pack_unpack(rows, "qc", boxes, "pp")
+    want = f0[0].copy()
+    src = f0[3].reshape(-1)
+    pos = 0
+    for (x0, y0, z0), (x1, y1, z1) in boxes:
+        n = (x1 - x0) * (y1 - y0) * (z1 - z0)
+        want[z0:z1, y0:y1, x0:x1] = src[pos:pos + n].reshape(
+            z1 - z0, y1 - y0, x1 - x0)
+        pos += n
+    assert np.array_equal(bits(d.download(0)), bits(want))
+    d.close()
+
+
 @pytest.mark.parametrize("case", ["a", "b"])
 @pytest.mark.parametrize("ranks,topo", [(1, (1, 1, 1)), (8, (2, 2, 2))])
 @pytest.mark.parametrize("kernel", [1, 0])
