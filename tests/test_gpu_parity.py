@@ -220,6 +220,26 @@ def test_prefetch_depths_are_bitwise_equal(H, VM):
             assert np.array_equal(bits(a), bits(b)), k
 
 
+def test_tile_shapes_are_bitwise_equal(H, VM):
+    """Every supported tile shape (32x16, 32x32, 64x16, 64x8) computes each
+    cell with the same expressions: a fused 30-step loop with a source gives
+    the same bits with all four.  The 52 x 42 x 36 compute cells leave
+    ragged tiles in x and y for every shape."""
+    w = VM.random_small((44, 50, 60), seed=15, steps=30)
+    amp = w.source_table()
+    out = []
+    for tx, ty in ((32, 16), (32, 32), (64, 16), (64, 8)):
+        say("tile", tx, ty)
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        assert d.set_tiling(tx, ty, 0)
+        d.run(30, w.src_index, amp, graph=True)
+        out.append(d.fields())
+        d.close()
+    for k in (1, 2, 3):
+        for a, b in zip(out[0], out[k]):
+            assert np.array_equal(bits(a), bits(b)), k
+
+
 def test_default_z_chunk_is_bitwise_equal_to_whole_columns(H, VM):
     """The default z chunk is chosen per launch (two tiles per CU): the
     192 x 192 x 112 compute cells of these arrays make 72 columns of 32 x 16
