@@ -1,5 +1,5 @@
 """Learned absorbing layer: training data from the propagator, training on the
-GPU, and the reflection score against a padded-domain judge.
+GPU, and the reflection score against a padded-domain reference run.
 
 The network and its kernels live in the library (include/dc_hip.h, "learned
 absorbing layer"); this module drives them:
@@ -13,11 +13,11 @@ absorbing layer"); this module drives them:
   domain's step (which sees zeros beyond its edge) equal the padded one's;
 * `train` runs Adam on those samples with the library's MFMA forward and
   backward kernels;
-* `reflection` runs one source in four ways -- the padded judge, the small
+* `reflection` runs one source in four ways -- the padded reference run, the small
   domain with the learned layer, the small domain with nothing at its edge,
   and the small domain whose outer cells are the reference's random-velocity
   shell (src/boundary.c:7-104) -- and scores each by the energy of its
-  difference from the judge at interior receivers.
+  difference from the padded run at interior receivers.
 
 The weights that ship with the package (`learned_layer_w6.npz`) were made by
 `python -m distributed-cube-average_amd.learned_boundary train`.
@@ -237,7 +237,7 @@ def _receivers(c: Case, gap=4, stride=4):
 
 
 def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
-    """Energy ratio of (run - judge) to judge at the interior receivers over
+    """Energy ratio of (run - padded) to padded at the interior receivers over
     `steps` steps, for the learned layer, no layer and random-velocity
     shells of the given widths (shells wider than the layer grow the domain
     outward so that the interior, the receivers and the source stay put)."""
@@ -256,12 +256,12 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
         return coef, idx, int(si)
 
     out = {}
-    # Judge: the padded domain.
+    # Reference: the padded domain.
     d = H.DeviceData(c.coef_pad, None, c.d, c.d, c.d, c.dt)
     _, idx, si = run(c.coef_pad, (0, 0, 0))
-    judge, _ = d.run_traces(steps, si, amp, idx, graph=graph)
+    padded, _ = d.run_traces(steps, si, amp, idx, graph=graph)
     d.close()
-    ej = float(np.sum(judge.astype(np.float64) ** 2))
+    ej = float(np.sum(padded.astype(np.float64) ** 2))
     lo_small = tuple(o - STENCIL for o in c.origin)
 
     def score(coef, lo, net):
@@ -276,7 +276,7 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
         dd.close()
         if not np.all(np.isfinite(tr)):
             return float("inf")
-        return float(np.sum((tr.astype(np.float64) - judge) ** 2) / ej)
+        return float(np.sum((tr.astype(np.float64) - padded) ** 2) / ej)
 
     small = c.small_coef()
     out["learned"] = score(small, lo_small, mlp)

@@ -270,7 +270,7 @@ def test_layer_step_matches_oracle(H, O, VM, mode, width):
 def test_shipped_layer_gains_over_the_layer_without_it(H, pkg):
     """The shipped 6-cell network against the same thin domain without it
     (zero wall) and with the reference's random-velocity shell of the same
-    width, scored against the padded judge on the case the weights were
+    width, scored against the padded reference run on the case the weights were
     evaluated on (128^3 window, seed 9001, 900 steps).  Measured when the
     weights were made: learned 0.083, shell 0.389, no layer 0.496."""
     import importlib
