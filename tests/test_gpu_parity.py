@@ -301,6 +301,53 @@ def test_halo_extract_insert_bitwise(H, O, VM):
     d.close()
 
 
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_arrays_past_2_gib(H, O, VM, kernel):
+    """Arrays of 1000 x 40 x 13,200 cells: with rows pitched to 1024 floats
+    each array spans 2.16 GB, so its last planes sit past 2^31 bytes, where a
+    signed 32-bit byte offset would wrap (the fast kernel's limit is 4 GiB).
+    A box in those planes matches the oracle (exact kernel bit for bit, fast
+    kernel within 1e-5) and nothing below the box changes.  Only the last 16
+    planes of the host arrays are written; the rest stay untouched zero
+    pages, so host memory stays small."""
+    sz, sy, sx = 13_200, 40, 1000
+    tail = 16
+    assert (sz - tail) * 1024 * sy * 4 > 2 ** 31
+    small = VM.random_small((tail, sy, sx), seed=21)
+    rng = np.random.default_rng(22)
+    coef, f0 = [], []
+    for c in small.coef:
+        a = np.zeros((sz, sy, sx), np.float32)
+        a[-tail:] = c
+        coef.append(a)
+    for _ in range(4):
+        a = np.zeros((sz, sy, sx), np.float32)
+        a[-tail:] = rng.standard_normal((tail, sy, sx)).astype(np.float32)
+        f0.append(a)
+    ref = [a[-tail:].copy() for a in f0]
+    O.propagate_f32((4, 4, 4), (sx - 4, sy - 4, 12), ref[0], ref[1], ref[2],
+                    ref[3], small.coef, small.dx, small.dy, small.dz, small.dt)
+    say("past 2 GiB, kernel", kernel)
+    d = H.DeviceData(coef, f0, small.dx, small.dy, small.dz, small.dt,
+                     kernel=kernel)
+    d.propagate((4, 4, sz - 12), (sx - 4, sy - 4, sz - 4))
+    inside = np.zeros((tail, sy, sx), bool)
+    inside[4:12, 4:sy - 4, 4:sx - 4] = True
+    for name, want, old in (("pp", ref[0], f0[0][-tail:]),
+                            ("qp", ref[2], f0[2][-tail:])):
+        got = d.extract_box((0, 0, sz - tail), (sx, sy, sz), name).reshape(
+            tail, sy, sx)
+        assert np.array_equal(bits(got[~inside]), bits(old[~inside]))
+        if kernel == 1:
+            assert np.array_equal(bits(got[inside]), bits(want[inside]))
+        else:
+            assert rel_l2(got[inside] - old[inside],
+                          want[inside] - old[inside]) < FAST_TOL
+        below = d.extract_box((0, 0, sz - tail - 4), (sx, sy, sz - tail), name)
+        assert not below.any()
+    d.close()
+
+
 def test_multi_box_pack_unpack_bitwise(H, VM):
     """Many boxes per launch (dcx_pack_boxes / dcx_unpack_boxes): faces one
     to four cells thin, an edge, a corner, one cell, an empty box, a ragged
