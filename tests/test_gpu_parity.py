@@ -224,7 +224,8 @@ def test_tile_shapes_are_bitwise_equal(H, VM):
     """Every supported tile shape (32x16, 32x32, 64x16, 64x8) computes each
     cell with the same expressions: a fused 30-step loop with a source gives
     the same bits with all four.  The 52 x 42 x 36 compute cells leave
-    ragged tiles in x and y for every shape."""
+    ragged tiles in x and y for every shape.  Other shapes and a negative z
+    chunk are refused and leave the tiling as it was."""
     w = VM.random_small((44, 50, 60), seed=15, steps=30)
     amp = w.source_table()
     out = []
@@ -232,6 +233,8 @@ def test_tile_shapes_are_bitwise_equal(H, VM):
         say("tile", tx, ty)
         d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
         assert d.set_tiling(tx, ty, 0)
+        for bad in ((16, 16, 0), (32, 8, 0), (128, 4, 0), (tx, ty, -1)):
+            assert not d.set_tiling(*bad)
         d.run(30, w.src_index, amp, graph=True)
         out.append(d.fields())
         d.close()
