@@ -633,6 +633,9 @@ __global__ void __launch_bounds__(TX *TY)
   }
 }
 
+// Tile shapes with a kernel instance: 512 threads (32x16, 64x8) or 1024
+// threads (32x32, 64x16; at 1024 threads a lane has 128 registers, and these
+// spill).
 bool fast_tiling_supported(int tx, int ty) {
   return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && (ty == 8 || ty == 16));
 }
