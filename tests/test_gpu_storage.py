@@ -67,3 +67,23 @@ def test_storage_round_trip(H, run40):
     for a, b in zip(mid, after):
         np.testing.assert_array_equal(a, b)
     dd.close()
+
+
+@pytest.mark.parametrize("form", [(0, 2), (1, 1), (1, 2), (2, 2)])
+def test_storage_forms_tile_shapes_bitwise_equal(H, run40, form):
+    """A compact form computes each cell with the same expressions whatever
+    the tile: 64x8 tiles (two apron cells per thread) give the bits of 32x16
+    tiles over the 40-step fixture loop."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    out = []
+    for tx, ty in ((32, 16), (64, 8)):
+        say("storage", form, "tile", tx, ty)
+        dd = H.DeviceData(coef, None, d, d, d, dt)
+        assert dd.set_tiling(tx, ty, 0)
+        assert dd.set_storage(*form)
+        dd.run(steps, src, amp, graph=True)
+        out.append(dd.fields())
+        dd.close()
+    for a, b in zip(*out):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                              np.ascontiguousarray(b).view(np.uint32))
