@@ -79,6 +79,44 @@ def octant_halo_boxes(sizes):
     return send, recv
 
 
+def reference_call_pattern(d, sizes, steps=50):
+    """The step as the reference's worker drives a backend
+    (src/worker.c:321-402): dc_propagate on the six 4-cell boundary slabs
+    (x, then y over x in [8, sx-8), then z over x, y in [8, s-8); low side
+    first), then on the interior [8, s-8)^3, then dc_device_swap_arrays.
+    Beside it, one dc_propagate over the whole compute region.  Wall-clock
+    ms per step between stream synchronisations, after one untimed step.
+    The struct runs with the library's default tiling, as a reference
+    worker's would (32 x 16 tiles, z chunk chosen per launch); the caller
+    restores its own tiling afterwards."""
+    assert d.set_tiling(32, 16, 0)
+    sx, sy, sz = sizes
+    S = 4
+    slabs = [((S, S, S), (2 * S, sy - S, sz - S)),
+             ((sx - 2 * S, S, S), (sx - S, sy - S, sz - S)),
+             ((2 * S, S, S), (sx - 2 * S, 2 * S, sz - S)),
+             ((2 * S, sy - 2 * S, S), (sx - 2 * S, sy - S, sz - S)),
+             ((2 * S, 2 * S, S), (sx - 2 * S, sy - 2 * S, 2 * S)),
+             ((2 * S, 2 * S, sz - 2 * S), (sx - 2 * S, sy - 2 * S, sz - S))]
+    interior = ((2 * S, 2 * S, 2 * S), (sx - 2 * S, sy - 2 * S, sz - 2 * S))
+    whole = ((S, S, S), (sx - S, sy - S, sz - S))
+    out = {}
+    for name, boxes in (("seven_calls", slabs + [interior]), ("one_call", [whole])):
+        for b in boxes:
+            d.propagate(*b)
+        d.swap()
+        d.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for b in boxes:
+                d.propagate(*b)
+            d.swap()
+        d.synchronize()
+        out[name + "_ms"] = round((time.perf_counter() - t0) * 1e3 / steps, 4)
+    out["steps"] = steps
+    return out
+
+
 def make_workload(VM, args):
     t = time.time()
     w = VM.layered_octant(seed=args.seed, local=args.local)
@@ -496,6 +534,13 @@ def main():
     log(f"[bench] stencil launch {kern_ms:.3f} ms -> {achieved:.0f} GB/s "
         f"algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
 
+    # The reference worker's seven dc_propagate calls per step, and one.
+    calls = reference_call_pattern(d, w.sizes)
+    d.set_tiling(*args.tiling)
+    log(f"[bench] dc_propagate as the reference's worker calls it: "
+        f"{calls['seven_calls_ms']:.3f} ms/step (seven boxes) vs "
+        f"{calls['one_call_ms']:.3f} ms/step (one box)")
+
     forms = None
     if not args.no_forms:
         forms = time_forms(H, w, d, args, amp, cells)
@@ -558,6 +603,7 @@ def main():
                    "device_ms_per_step": round(dev_ms / args.steps, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_call_pattern": calls,
         "forms": forms,
         "batch": batch,
         "learned_layer": learned,

@@ -160,7 +160,7 @@ struct Plan {
         a.zchunk = zchunk;
         a.deep = deep;
         a.xcd_remap = 0;
-        a.gx = (int)((box.x1 - box.x0 + 31) / 32);
+        a.gx = (int)((box.x1 - tile_origin_x(box.x0) + 31) / 32);
         a.gy = (int)((box.y1 - box.y0 + 15) / 16);
         a.src_z = -1;
         a.src_xy = -1;

@@ -189,6 +189,15 @@ constexpr uint64_t kMagic = 0x64635f6869703031ull;  // "dc_hip01"
 constexpr int64_t kRowAlign = 32;  // floats per 128-byte cache line
 constexpr int64_t kPad = (kRowAlign - STENCIL % kRowAlign) % kRowAlign;
 
+// x at which the fast kernel's tile grid starts for a box that starts at x0:
+// the largest x <= x0 whose element opens a 128-byte line of the pitched
+// arrays (x = STENCIL + 32k), so that each 32-element tile row lies in one
+// line whatever the box.  The kernel masks the cells before x0.  Every box
+// starts at x0 >= STENCIL, so the tile's apron starts at x >= 0.
+__host__ __device__ inline int64_t tile_origin_x(int64_t x0) {
+  return x0 >= STENCIL ? x0 - ((x0 - STENCIL) & (kRowAlign - 1)) : x0;
+}
+
 // Device offset of the caller's linear index x + y*sx + z*sx*sy.
 int64_t device_index(const State *st, int64_t logical);
 

@@ -45,7 +45,7 @@ static float *field_of(const State *st, const float *token) {
 static int auto_zchunk(const State *st, const Box &box) {
   const int64_t nx = box.x1 - box.x0, ny = box.y1 - box.y0, nz = box.z1 - box.z0;
   if (nx <= 0 || ny <= 0 || nz <= 0) return 512;
-  const int64_t cols = ((nx + st->tile_x - 1) / st->tile_x) *
+  const int64_t cols = ((box.x1 - tile_origin_x(box.x0) + st->tile_x - 1) / st->tile_x) *
                        ((ny + st->tile_y - 1) / st->tile_y);
   const int64_t target = 2 * (int64_t)(st->n_cu > 0 ? st->n_cu : 256);
   const int64_t chunks = (target + cols - 1) / cols;

@@ -317,14 +317,14 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   const int ys = (int)a.ys;
   // Tile b of the volume: x fastest, then y, then z chunk.
   const int bx = b % a.gx, by = (b / a.gx) % a.gy, bz = b / (a.gx * a.gy);
-  const int tx0 = (int)a.box.x0 + bx * TX;
+  const int tx0 = (int)tile_origin_x(a.box.x0) + bx * TX;
   const int ty0 = (int)a.box.y0 + by * TY;
   // Last coordinate any updated cell reads; loads are clamped to it so that a
   // tile overhanging the box never leaves the array.
   const int xl = (int)a.box.x1 + 3, yl = (int)a.box.y1 + 3;
 
   const int x = tx0 + tx, y = ty0 + ty;
-  const bool act = x < a.box.x1 && y < a.box.y1;
+  const bool act = x >= a.box.x0 && x < a.box.x1 && y < a.box.y1;
   // In-plane offset of this thread's cell (one plane holds < 2^31 cells) and
   // its byte offsets in field and coefficient arrays.
   const int own = (x < xl ? x : xl) + (y < yl ? y : yl) * ys;
@@ -676,7 +676,7 @@ static void launch_fast_tile(const StepArgs &b, dim3 grid, dim3 block,
 }
 
 int64_t fast_tiles(const StepArgs &a, int tile_x, int tile_y) {
-  const int64_t nx = a.box.x1 - a.box.x0, ny = a.box.y1 - a.box.y0,
+  const int64_t nx = a.box.x1 - tile_origin_x(a.box.x0), ny = a.box.y1 - a.box.y0,
                 nz = a.box.z1 - a.box.z0;
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
   return ((nx + tile_x - 1) / tile_x) * ((ny + tile_y - 1) / tile_y) *
@@ -739,7 +739,7 @@ void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
              (long long)(a.box.z1 + 4), (long long)a.zs);
   const int64_t gz = (nz + a.zchunk - 1) / a.zchunk;
   StepArgs b = a;
-  b.gx = (int)((nx + tile_x - 1) / tile_x);
+  b.gx = (int)((a.box.x1 - tile_origin_x(a.box.x0) + tile_x - 1) / tile_x);
   b.gy = (int)((ny + tile_y - 1) / tile_y);
   const int64_t nblocks = (int64_t)b.gx * b.gy * gz;
   if (nblocks >= (int64_t(1) << 31)) DCX_FAIL("too many tiles");
