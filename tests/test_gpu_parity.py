@@ -243,6 +243,42 @@ def test_tile_shapes_are_bitwise_equal(H, VM):
             assert np.array_equal(bits(a), bits(b)), k
 
 
+@pytest.mark.parametrize("kernel", [1, 0])
+def test_reference_worker_seven_calls_equal_one_call(H, VM, kernel):
+    """The reference's worker updates a step as six 4-cell boundary slabs,
+    then the interior (src/worker.c:321-402).  The fast kernel starts each
+    box's tile grid on the last 128-byte line boundary at or before its x0
+    (the interior and the y and z slabs start at x = 8) and masks the cells
+    before x0.  Each cell's update is the same, so 20 steps with a source
+    give the bits of one whole-region call per step."""
+    w = VM.random_small((44, 50, 60), seed=16, steps=20)
+    amp = w.source_table()
+    sx, sy, sz = w.sizes
+    S = 4
+    slabs = [((S, S, S), (2 * S, sy - S, sz - S)),
+             ((sx - 2 * S, S, S), (sx - S, sy - S, sz - S)),
+             ((2 * S, S, S), (sx - 2 * S, 2 * S, sz - S)),
+             ((2 * S, sy - 2 * S, S), (sx - 2 * S, sy - S, sz - S)),
+             ((2 * S, 2 * S, S), (sx - 2 * S, sy - 2 * S, 2 * S)),
+             ((2 * S, 2 * S, sz - 2 * S), (sx - 2 * S, sy - 2 * S, sz - S))]
+    interior = ((2 * S,) * 3, (sx - 2 * S, sy - 2 * S, sz - 2 * S))
+    whole = ((S,) * 3, (sx - S, sy - S, sz - S))
+    out = []
+    for boxes in (slabs + [interior], [whole]):
+        say("worker calls", len(boxes), "kernel", kernel)
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt, kernel=kernel)
+        for it in range(20):
+            d.add_source(w.src_index, float(amp[it]))
+            for b in boxes:
+                d.propagate(*b)
+            d.swap()
+        out.append(d.fields())
+        d.close()
+    assert np.abs(out[1][1]).max() > 0
+    for a, b in zip(*out):
+        assert np.array_equal(bits(a), bits(b))
+
+
 def test_default_z_chunk_is_bitwise_equal_to_whole_columns(H, VM):
     """The default z chunk is chosen per launch (two tiles per CU): the
     192 x 192 x 112 compute cells of these arrays make 72 columns of 32 x 16
