@@ -70,6 +70,12 @@ struct StepArgs {
   void *of[2];
   void *nf[2];
   const void *cf[NCOEF];
+  // Layout of the arrays cf points to: the pitched layout of the fields
+  // (cw == 0: offset x + y*ys + z*zs), or a dense copy of a band of cw cells
+  // in x that starts at x = cx0 (offset (x - cx0) + y*cw + z*czs, with x
+  // clamped into the band).
+  int cx0, cw;
+  int64_t czs;
   float c_lo[NCOEF], c_st[NCOEF];  // coefficient decode parameters
   float dscale;                    // time-difference units per field unit
   int fstore, cstore;              // DCX_STORE_*, DCX_COEF_*
@@ -133,6 +139,12 @@ struct State {
   float *field_alloc[4];
   float *field_mem[4];   // field_alloc + kPad
   float *coef_alloc;     // ten arrays of alloc_floats each
+  // Dense copies of the ten FP32 coefficient arrays over the x bands
+  // [STENCIL, 2*STENCIL) and [sx - 2*STENCIL, sx - STENCIL), all y and z:
+  // [side][coefficient][z][y][STENCIL].  The reference worker's x boundary
+  // slabs read them instead of 16 bytes from every 128-byte line of the
+  // pitched arrays.  Null when sx < 4*STENCIL.
+  float *xband;
   float *scratch;
   size_t scratch_n;
   float *stage;  // device staging for the ABI halo copies

@@ -115,9 +115,27 @@ void launch_stencil(const State *st, const StepArgs &a) {
     if (st->fstore != DCX_STORE_F32 || st->cstore != DCX_COEF_F32)
       DCX_FAIL("the exact kernel runs on FP32 storage only");
     launch_exact(a, st->stream);
+    return;
   }
-  else
-    launch_fast(a, st->tile_x, st->tile_y, st->stream);
+  // A box inside one of the two 4-cell x bands (the reference worker's x
+  // boundary slabs) reads its FP32 coefficients from the band's dense copy.
+  if (st->xband && a.cstore == DCX_COEF_F32) {
+    const int64_t band = STENCIL * st->sizes[1] * st->sizes[2];
+    for (int side = 0; side < 2; ++side) {
+      const int64_t bx0 = side ? st->sizes[0] - 2 * STENCIL : STENCIL;
+      if (a.box.x0 >= bx0 && a.box.x1 <= bx0 + STENCIL) {
+        StepArgs b = a;
+        for (int k = 0; k < NCOEF; ++k)
+          b.cf[k] = st->xband + ((size_t)side * NCOEF + k) * (size_t)band;
+        b.cx0 = (int)bx0;
+        b.cw = STENCIL;
+        b.czs = STENCIL * st->sizes[1];
+        launch_fast(b, st->tile_x, st->tile_y, st->stream);
+        return;
+      }
+    }
+  }
+  launch_fast(a, st->tile_x, st->tile_y, st->stream);
 }
 
 void adopt_stream(State *st, hipStream_t s) {
@@ -348,6 +366,21 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
     dc[k] = st->coef_alloc + (size_t)k * st->alloc_floats + kPad;
     copy_in(st, dc[k], hc[k], tmp);
   }
+  // Dense copies of the coefficients over the two x bands (one 2-D copy of
+  // 16-byte rows per array and band).
+  if (st->sizes[0] >= 4 * STENCIL) {
+    const size_t band = (size_t)STENCIL * st->sizes[1] * st->sizes[2];
+    DCX_CHECK(hipMalloc(&st->xband, 2 * NCOEF * band * sizeof(float)));
+    for (int side = 0; side < 2; ++side) {
+      const int64_t bx0 = side ? st->sizes[0] - 2 * STENCIL : STENCIL;
+      for (int k = 0; k < NCOEF; ++k)
+        DCX_CHECK(hipMemcpy2DAsync(
+            st->xband + ((size_t)side * NCOEF + k) * band,
+            STENCIL * sizeof(float), dc[k] + bx0, st->ys * sizeof(float),
+            STENCIL * sizeof(float), st->sizes[1] * st->sizes[2],
+            hipMemcpyDeviceToDevice, st->stream));
+    }
+  }
   DCX_CHECK(hipStreamSynchronize(st->stream));
   DCX_CHECK(hipFree(tmp));
 
@@ -371,6 +404,7 @@ void dc_device_data_free(dc_device_data *data) {
   DCX_CHECK(hipStreamSynchronize(st->stream));
   for (int k = 0; k < 4; ++k) DCX_CHECK(hipFree(st->field_alloc[k]));
   DCX_CHECK(hipFree(st->coef_alloc));
+  if (st->xband) DCX_CHECK(hipFree(st->xband));
   if (st->scratch) DCX_CHECK(hipFree(st->scratch));
   if (st->stage) DCX_CHECK(hipFree(st->stage));
   if (st->halo_tab) DCX_CHECK(hipFree(st->halo_tab));

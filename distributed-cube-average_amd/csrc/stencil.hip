@@ -329,7 +329,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // its byte offsets in field and coefficient arrays.
   const int own = (x < xl ? x : xl) + (y < yl ? y : yl) * ys;
   const uint32_t ob = (uint32_t)own * F::kBytes;
-  const uint32_t obc = (uint32_t)own * Cf::kBytes;
+  // Coefficient offset: the pitched layout, or a dense x band (cw > 0).
+  const int ownc = a.cw > 0 ? min(max(x - a.cx0, 0), a.cw - 1) + (y < yl ? y : yl) * a.cw
+                            : own;
+  const uint32_t obc = (uint32_t)ownc * Cf::kBytes;
   const int lo = (ty + 4) * PW + tx + 4;  // own cell in the LDS plane
   const int ld = (ty + 4) * TX + tx;      // own cell in the D1x plane
 
@@ -382,7 +385,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // Byte offset of plane k: every array is smaller than 4 GiB (checked at
   // launch), so one 32-bit offset per plane serves all arrays of a kind.
   const uint32_t zsb = (uint32_t)zs * F::kBytes;
-  const uint32_t zsbc = (uint32_t)zs * Cf::kBytes;
+  const uint32_t zsbc = (uint32_t)(a.cw > 0 ? a.czs : zs) * Cf::kBytes;
   const int zb = (int)a.box.z0 + bz * a.zchunk;
   const int ze = zb + a.zchunk < (int)a.box.z1 ? zb + a.zchunk : (int)a.box.z1;
 
