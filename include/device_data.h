@@ -9,7 +9,10 @@
  *  - pp/pc/qp/qc in the returned struct are device addresses the caller only
  *    passes back (as the from/to array of the halo calls).  vpz and vsv are
  *    NULL: the kernels read the ten precomputed arrays, as the reference's
- *    OpenMP path does (src/openmp_propagate.c:22-27).
+ *    OpenMP path does (src/openmp_propagate.c:22-27).  The precomp_vars
+ *    pointers are device arrays the library only reads; it also keeps copies
+ *    of their 4-cell x bands made at init, so they are not to be written
+ *    afterwards.
  *  - Every call is ordered on one HIP stream owned by the struct.  Calls that
  *    hand data back to host memory (dc_device_data_get_results,
  *    dc_device_extract_halo_face) return only once it is there; insert returns
