@@ -475,6 +475,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=100)
     ap.add_argument("--no-forms", action="store_true")
+    ap.add_argument("--no-calls", action="store_true",
+                    help="skip the reference worker's seven-call step "
+                         "(reference_call_pattern is then null)")
     ap.add_argument("--batch-shots", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=800)
     ap.add_argument("--no-batch", action="store_true")
@@ -535,11 +538,13 @@ def main():
         f"algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
 
     # The reference worker's seven dc_propagate calls per step, and one.
-    calls = reference_call_pattern(d, w.sizes)
-    d.set_tiling(*args.tiling)
-    log(f"[bench] dc_propagate as the reference's worker calls it: "
-        f"{calls['seven_calls_ms']:.3f} ms/step (seven boxes) vs "
-        f"{calls['one_call_ms']:.3f} ms/step (one box)")
+    calls = None
+    if not args.no_calls:
+        calls = reference_call_pattern(d, w.sizes)
+        d.set_tiling(*args.tiling)
+        log(f"[bench] dc_propagate as the reference's worker calls it: "
+            f"{calls['seven_calls_ms']:.3f} ms/step (seven boxes) vs "
+            f"{calls['one_call_ms']:.3f} ms/step (one box)")
 
     forms = None
     if not args.no_forms:
