@@ -1,0 +1,58 @@
+"""The C ABI's failure behaviour, as the reference's backends have it
+(src/device_data.cu:10-16, src/cuda_propagate.cu:63-68): a message on stderr
+and exit status 1.  Every case is a bad argument the library checks on the
+host before any device work for that call, so no kernel runs on bad input.
+Each case runs in a process of its own, one after the other.
+"""
+import os
+import subprocess
+import sys
+import textwrap
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PRELUDE = textwrap.dedent("""
+    import ctypes as C, importlib, sys
+    import numpy as np
+    sys.path.insert(0, {repo!r})
+    H = importlib.import_module("distributed-cube-average_amd").dc_hip
+    VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
+    w = VM.random_small((20, 22, 24), seed=1)   # sizes (x, y, z) = (24, 22, 20)
+    d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+    d.synchronize()
+    print("ready", flush=True)
+""")
+
+# (statement, text the message must hold)
+CASES = {
+    "box below the stencil": ("d.propagate((3, 4, 4), (10, 10, 10))", "must lie in"),
+    "box past the array": ("d.propagate((4, 4, 4), (21, 10, 10))", "must lie in"),
+    "source out of range": ("d.add_source(24 * 22 * 20, 1.0)", "out of range"),
+    "unknown kernel": ("H.load().dcx_set_kernel(d.ptr, 7)", "unknown kernel"),
+    "sizes differ": (
+        "H.load().dc_propagate(H._SZ3(4, 4, 4), H._SZ3(8, 8, 8), "
+        "H._SZ3(24, 22, 21), H._I3(0, 0, 0), H._I3(1, 1, 1), d.ptr, "
+        "20.0, 20.0, 20.0, 1e-3)", "differ"),
+    "not one of the fields": (
+        "buf = np.zeros(64, np.float32); "
+        "H.load().dc_device_extract_halo_face(d.ptr, H.fp(buf), "
+        "H._SZ3(4, 4, 4), H._SZ3(8, 8, 8), H._SZ3(24, 22, 20), "
+        "C.cast(C.c_void_p(4096), H._FP))", "not one of"),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_bad_call_prints_and_exits_1(case):
+    stmt, text = CASES[case]
+    code = PRELUDE.format(repo=REPO) + stmt + '\nprint("not reached", flush=True)\n'
+    print("error case:", case, file=sys.stderr, flush=True)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                       text=True, timeout=120)
+    assert "ready" in r.stdout, r.stderr
+    assert r.returncode == 1, (r.returncode, r.stderr)
+    assert "not reached" not in r.stdout
+    assert "[dc_hip] error:" in r.stderr and text in r.stderr, r.stderr
