@@ -101,8 +101,9 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
     a.dscale = st->dscale;
   }
   for (int k = 0; k < NCOEF; ++k) {
-    a.cf[k] = st->cstore == DCX_COEF_F32
-                  ? (const void *)a.coef.c[k]
+    const bool f32 = st->cstore == DCX_COEF_F32 ||
+                     (st->cstore == DCX_COEF_A16 && k >= V2PX);
+    a.cf[k] = f32 ? (const void *)a.coef.c[k]
                   : (const void *)(st->C16 + (size_t)k * st->alloc_floats);
     a.c_lo[k] = st->c_lo[k];
     a.c_st[k] = st->c_st[k];

@@ -272,6 +272,23 @@ template <> struct CoefIO<DCX_COEF_U16> {
   }
 };
 
+// Angle terms as 16-bit affine codes, v^2 terms in FP32.  Offsets are formed
+// for 2-byte elements and doubled for the FP32 arrays (every array is below
+// 4 GiB, checked at launch).
+template <> struct CoefIO<DCX_COEF_A16> {
+  static constexpr uint32_t kBytes = 2;
+  static __device__ __forceinline__ uint32_t ldr(const StepArgs &a, int k,
+                                                 uint32_t o) {
+    if (k < V2PX) return ldb<uint16_t>(a.cf[k], o);
+    return ldb<uint32_t>(a.cf[k], 2u * o);
+  }
+  static __device__ __forceinline__ float dec(const StepArgs &a, int k,
+                                              uint32_t r) {
+    if (k < V2PX) return fmaf((float)r, a.c_st[k], a.c_lo[k]);
+    return __uint_as_float(r);
+  }
+};
+
 // Per-step global inputs, raw: coefficients of the staged plane (in-plane
 // part of H1) and of the output plane, and the output plane's old state
 // (previous level for leapfrog storage, time difference otherwise).
@@ -669,6 +686,7 @@ static void launch_fast_tile(const StepArgs &b, dim3 grid, dim3 block,
   DCX_FORM(DCX_STORE_F32, DCX_COEF_F32)
   DCX_FORM(DCX_STORE_F32, DCX_COEF_F16)
   DCX_FORM(DCX_STORE_F32, DCX_COEF_U16)
+  DCX_FORM(DCX_STORE_F32, DCX_COEF_A16)
   DCX_FORM(DCX_STORE_F16, DCX_COEF_F32)
   DCX_FORM(DCX_STORE_F16, DCX_COEF_F16)
   DCX_FORM(DCX_STORE_F16, DCX_COEF_U16)
@@ -725,6 +743,7 @@ void launch_fast_batch(const StepArgs *shots, const int64_t *prefix,
   }
   DCX_BFORM(DCX_STORE_F32, DCX_COEF_F32)
   DCX_BFORM(DCX_STORE_F32, DCX_COEF_U16)
+  DCX_BFORM(DCX_STORE_F32, DCX_COEF_A16)
   DCX_BFORM(DCX_STORE_F16, DCX_COEF_F16)
   DCX_BFORM(DCX_STORE_F16, DCX_COEF_U16)
 #undef DCX_BFORM

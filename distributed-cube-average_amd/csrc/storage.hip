@@ -202,9 +202,10 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
                      field_store == DCX_STORE_F16 ||
                      field_store == DCX_STORE_BF16;
   const bool cs_ok = coef_store == DCX_COEF_F32 || coef_store == DCX_COEF_F16 ||
-                     coef_store == DCX_COEF_U16;
+                     coef_store == DCX_COEF_U16 || coef_store == DCX_COEF_A16;
   if (!fs_ok || !cs_ok) return -1;
   if (field_store == DCX_STORE_BF16 && coef_store == DCX_COEF_F32) return -1;
+  if (coef_store == DCX_COEF_A16 && field_store != DCX_STORE_F32) return -1;
   DCX_CHECK(hipStreamSynchronize(st->stream));
 
   // Bring the fields back to FP32 first if they are compact.
@@ -254,7 +255,10 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
     float *d_mm = nullptr;
     DCX_CHECK(hipMalloc(&d_mm, 4 * kConvBlocks * sizeof(float)));
     std::vector<float> mm(4 * kConvBlocks);
-    for (int k = 0; k < NCOEF; ++k) {
+    // DCX_COEF_A16 codes the angle terms only; its v^2 terms stay in the
+    // FP32 arrays.
+    const int ncoded = coef_store == DCX_COEF_A16 ? V2PX : NCOEF;
+    for (int k = 0; k < ncoded; ++k) {
       hipLaunchKernelGGL(k_minmax, dim3(kConvBlocks), dim3(256), 0, st->stream,
                          whole(cs[k]), st->alloc_floats, d_mm);
       DCX_CHECK(hipGetLastError());
@@ -317,6 +321,10 @@ extern "C" int dcx_bytes_per_cell(const dc_device_data *data) {
   const State *st = state_of(data);
   const int fb = st->fstore == DCX_STORE_F32 ? 4 : 2;
   const int cb = st->cstore == DCX_COEF_F32 ? 4 : 2;
+  if (st->cstore == DCX_COEF_A16) {
+    // Angle terms 2 B each, v^2 terms 4 B each, FP32 leapfrog fields.
+    return 6 * fb + V2PX * 2 + (NCOEF - V2PX) * 4;
+  }
   // Leapfrog: read pc, qc, pp, qp, write pp, qp.  Time difference: read P and
   // D, write P and D, per field.
   const int fields = st->fstore == DCX_STORE_F32 ? 6 * fb : 8 * fb;

@@ -78,9 +78,11 @@ void dcx_upload(dc_device_data *data, int which, const float *host);
  *  DCX_COEF_F16    the ten arrays as half, each scaled by a power of two.
  *  DCX_COEF_U16    16-bit codes per array: affine over [min,max] for the six
  *                  angle terms, affine in log2 for the four v^2 terms (code
- *                  0 = exactly zero); ~14x finer than half on v^2. */
+ *                  0 = exactly zero); ~14x finer than half on v^2.
+ *  DCX_COEF_A16    the six angle terms as DCX_COEF_U16 codes them, the four
+ *                  v^2 terms in FP32 (FP32 field storage only). */
 enum { DCX_STORE_F32 = 0, DCX_STORE_F16 = 1, DCX_STORE_BF16 = 2 };
-enum { DCX_COEF_F32 = 0, DCX_COEF_F16 = 1, DCX_COEF_U16 = 2 };
+enum { DCX_COEF_F32 = 0, DCX_COEF_F16 = 1, DCX_COEF_U16 = 2, DCX_COEF_A16 = 3 };
 
 /* Convert the struct to the given storage (encoding the current fields and
  * coefficients; field_scale_log2 sets the field scale 2^k for half storage).
