@@ -1,8 +1,9 @@
 """Compact storage forms (dcx_set_storage) against the FP64 oracle.
 
-FP32 fields with 16-bit coefficient codes must stay within 1e-4 of FP64 on the
-40-step fixture; half/bfloat16 field storage is a precision trade-off whose
-error is printed and bounded loosely here (the study at 1000-4000 steps is in
+FP32 fields with 16-bit coefficient codes (all ten, or the six angle terms
+with the v^2 terms in FP32) must stay within 1e-4 of FP64 on the 40-step
+fixture; half/bfloat16 field storage is a precision trade-off whose error is
+printed and bounded loosely here (the study at 1000-4000 steps is in
 bench.py --precision-study).
 """
 import sys
