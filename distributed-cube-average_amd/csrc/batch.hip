@@ -150,8 +150,15 @@ struct Plan {
   void upload(int zchunk, int deep) {
     std::vector<StepArgs> args[2];
     std::vector<int64_t> prefix(n + 1, 0);
+    const State *s0 = b.shots[0];
     for (int k = 0; k < n; ++k) {
       State *st = b.shots[k];
+      // The entry points that could change these refuse batch shots; the
+      // launch picks one kernel for all shots from shot 0's form.
+      if (st->fstore != s0->fstore || st->cstore != s0->cstore ||
+          st->kernel != DCX_KERNEL_FAST || st->halo_nuser != 0)
+        DCX_FAIL("batch shot %d no longer matches the batch's storage form, "
+                 "kernel or halo setting", k);
       const Box box{STENCIL, STENCIL, STENCIL, st->sizes[0] - STENCIL,
                     st->sizes[1] - STENCIL, st->sizes[2] - STENCIL};
       const float *q = d4 + 4 * k;

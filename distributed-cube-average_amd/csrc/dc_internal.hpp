@@ -118,8 +118,11 @@ struct BoxCopy {
 // boxes is a device array of nbox entries; total = sum of box volumes.
 void launch_box_copy(const BoxCopy *boxes, int nbox, int64_t total, float *buf,
                      int dir, hipStream_t s);
-// Single box, passed by value.
+// Single box of any volume, passed by value.
 void launch_box_copy1(const BoxCopy &box, float *buf, int dir, hipStream_t s);
+// Fails (message, exit 1) for a box launch_box_copy cannot take: its in-box
+// coordinates are 32-bit, so each box must hold fewer than 2^32 cells.
+void check_multi_box(const BoxCopy &b);
 
 // Library state behind a dc_device_data pointer.  `abi` must stay first: the
 // caller holds &state->abi.
@@ -215,6 +218,15 @@ int64_t device_index(const State *st, int64_t logical);
 
 State *state_of(dc_device_data *d);
 const State *state_of(const dc_device_data *d);
+
+// Host checks of the reference ABI's box calls (each fails with a message and
+// exit 1): the caller's sizes equal the struct's; the token names one of the
+// struct's four fields (returned as its device array); the box [start, end)
+// lies in the local grid (returned as a copy record of `arr`, offset 0).
+void check_sizes(const State *st, const size_t sizes[3]);
+float *field_of(const State *st, const float *token);
+BoxCopy box_of(const State *st, const size_t start[3], const size_t end[3],
+               float *arr);
 
 // Build StepArgs for the current pointer state of `st` over `box`.
 StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,

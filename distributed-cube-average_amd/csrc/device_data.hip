@@ -21,7 +21,7 @@ const State *state_of(const dc_device_data *d) {
   return state_of(const_cast<dc_device_data *>(d));
 }
 
-static void check_sizes(const State *st, const size_t sizes[3]) {
+void check_sizes(const State *st, const size_t sizes[3]) {
   for (int k = 0; k < 3; ++k)
     if ((int64_t)sizes[k] != st->sizes[k])
       DCX_FAIL("sizes (%zu,%zu,%zu) differ from the initialised (%lld,%lld,%lld)",
@@ -30,7 +30,7 @@ static void check_sizes(const State *st, const size_t sizes[3]) {
 }
 
 // Which device array a caller's field token names.
-static float *field_of(const State *st, const float *token) {
+float *field_of(const State *st, const float *token) {
   const dc_device_data &d = st->abi;
   if (token == d.pp || token == d.pc || token == d.qp || token == d.qc)
     return const_cast<float *>(token);
@@ -163,6 +163,38 @@ static void set_source(const State *st, StepArgs &a, long long index,
   a.src_amp = st->fstore == DCX_STORE_F32 ? amp : amp * st->fscale;
 }
 
+// Whether the cell of logical index `index` lies in a send box of the
+// registered loopback.  The reference's worker adds step it+1's source to pc
+// after the halos of step it have been sent and received
+// (src/worker.c:466-511), so a neighbour never receives it.  The fused step
+// adds it as the stencil writes the new field, before the loopback packs
+// that field; for a source cell inside a send box the step therefore adds it
+// after the loopback instead (add_deferred_source).
+static bool source_in_send_box(const State *st, long long index) {
+  if (index < 0 || st->halo_nuser == 0) return false;
+  const int64_t plane = st->sizes[0] * st->sizes[1];
+  const int64_t c[3] = {index % st->sizes[0], (index % plane) / st->sizes[0],
+                        index / plane};
+  for (int k = 0; k < st->halo_nuser; ++k) {
+    const size_t *u = st->halo_user + 12 * k;
+    bool in = true;
+    for (int a = 0; a < 3; ++a)
+      in = in && c[a] >= (int64_t)u[a] && c[a] < (int64_t)u[3 + a];
+    if (in) return true;
+  }
+  return false;
+}
+
+// Add a source amplitude to the current fields (pc, qc) after a step's
+// loopback: dc_device_add_source's additions.
+static void add_deferred_source(State *st, long long index, float amp) {
+  if (st->fstore == DCX_STORE_F32)
+    launch_add_source(st->abi.pc, st->abi.qc, device_index(st, index), amp,
+                      st->stream);
+  else
+    storage_add_source(st, device_index(st, index), amp);
+}
+
 // Advance the struct one step: swap the leapfrog pointers, or flip the
 // current level of compact storage.
 static void advance(State *st) {
@@ -243,7 +275,7 @@ static int select_device(const dc_process_t *p) {
   return same % count;
 }
 
-static BoxCopy box_of(const State *st, const size_t start[3],
+BoxCopy box_of(const State *st, const size_t start[3],
                       const size_t end[3], float *arr) {
   for (int k = 0; k < 3; ++k)
     if (start[k] > end[k] || (int64_t)end[k] > st->sizes[k])
@@ -292,6 +324,7 @@ void build_halo_tables(State *st) {
           DCX_FAIL("send and receive box %d differ in shape", k);
         pb.esize = ub.esize = f32 ? 4 : 2;
         pb.off = ub.off = total;
+        check_multi_box(pb);
         tab[(size_t)(2 * par) * nb + 2 * k + f] = pb;
         tab[(size_t)(2 * par + 1) * nb + 2 * k + f] = ub;
       }
@@ -524,6 +557,9 @@ int dcx_set_kernel(dc_device_data *data, int kernel) {
   State *st = state_of(data);
   if (kernel != DCX_KERNEL_FAST && kernel != DCX_KERNEL_EXACT)
     DCX_FAIL("unknown kernel %d", kernel);
+  if (st->batch && kernel != DCX_KERNEL_FAST)
+    DCX_FAIL("dcx_set_kernel on a batch shot: batched launches run the fast "
+             "kernel only (dcx_batch_free first)");
   const int prev = st->kernel;
   st->kernel = kernel;
   return prev;
@@ -617,13 +653,15 @@ void fused_step(State *st, float dx, float dy, float dz, float dt,
                 long long src_index, float src_next, int add_next,
                 void (*after_stencil)(void *), void *ctx) {
   StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
-  if (add_next) set_source(st, a, src_index, src_next);
+  const bool defer = add_next && source_in_send_box(st, src_index);
+  if (add_next && !defer) set_source(st, a, src_index, src_next);
   layer_infer(st, a);
   launch_stencil(st, a);
   if (after_stencil) after_stencil(ctx);
   layer_apply(st);
   halo_loopback(st);
   advance(st);
+  if (defer) add_deferred_source(st, src_index, src_next);
 }
 }  // namespace dcx
 
@@ -707,6 +745,7 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
   dc_device_data *data = &st->abi;
   const bool src = src_index >= 0 && src_amp != nullptr;
   if (src && src_index >= st->n) DCX_FAIL("source index out of range");
+  const bool defer = src && source_in_send_box(st, src_index);
   // Step it writes the fields that step it+1 reads as pc/qc; the source of
   // step it+1 is added as they are written (see dcx_step).
   auto enqueue = [&](std::vector<hipEvent_t> *evs) {
@@ -719,7 +758,8 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
     }
     for (unsigned it = 0; it < n_steps; ++it) {
       StepArgs a = make_step_args(st, full_box(st), dx, dy, dz, dt);
-      if (src && it + 1 < n_steps) set_source(st, a, src_index, src_amp[it + 1]);
+      if (src && !defer && it + 1 < n_steps)
+        set_source(st, a, src_index, src_amp[it + 1]);
       layer_infer(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it], st->stream));
       launch_stencil(st, a);
@@ -728,6 +768,8 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
       halo_loopback(st);
       advance(st);
       if (tc) gather(st, *tc, it);
+      if (defer && it + 1 < n_steps)
+        add_deferred_source(st, src_index, src_amp[it + 1]);
     }
   };
   float ms = 0.f;
@@ -839,6 +881,7 @@ static void boxes_of(State *st, int nbox, const size_t *starts,
   total = 0;
   for (int k = 0; k < nbox; ++k) {
     out[k] = box_of(st, starts + 3 * k, ends + 3 * k, arr);
+    check_multi_box(out[k]);
     out[k].off = total;
     total += out[k].nx * out[k].ny * out[k].nz;
   }
@@ -863,6 +906,9 @@ void dcx_set_halo_loopback(dc_device_data *data, int nbox,
                            const size_t *send_starts, const size_t *send_ends,
                            const size_t *recv_starts, const size_t *recv_ends) {
   State *st = state_of(data);
+  if (st->batch)
+    DCX_FAIL("dcx_set_halo_loopback on a batch shot: batched launches carry "
+             "no halo exchange (dcx_batch_free first)");
   std::free(st->halo_user);
   st->halo_user = nullptr;
   st->halo_nuser = nbox > 0 ? nbox : 0;

@@ -127,17 +127,17 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// One box of any volume (the whole-allocation copy of dcx_copy_field can pass
+// 2^32 elements): in-box coordinates in 64-bit arithmetic.
 __global__ void __launch_bounds__(256)
     k_box_copy1(const BoxCopy b, float *__restrict__ buf, int dir) {
   const int64_t total = b.nx * b.ny * b.nz;
-  const uint32_t nx = (uint32_t)b.nx, ny = (uint32_t)b.ny;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t k = (uint32_t)e;
-    const uint32_t r = k / nx;
-    const int64_t x = k - r * nx;
-    const int64_t z = r / ny;
-    const int64_t y = r - (uint32_t)z * ny;
+    const int64_t r = e / b.nx;
+    const int64_t x = e - r * b.nx;
+    const int64_t z = r / b.ny;
+    const int64_t y = r - z * b.ny;
     const int64_t i = (b.x0 + x) + (b.y0 + y) * b.ys + (b.z0 + z) * b.zs;
     copy_elem(b, i, buf, b.off + e, dir);
   }
@@ -147,6 +147,13 @@ static unsigned copy_blocks(int64_t total) {
   int64_t g = (total + 255) / 256;
   if (g > 8192) g = 8192;
   return (unsigned)(g < 1 ? 1 : g);
+}
+
+void check_multi_box(const BoxCopy &b) {
+  if (b.nx * b.ny * b.nz >= (int64_t(1) << 32))
+    DCX_FAIL("box of %lld x %lld x %lld cells: a multi-box copy takes boxes "
+             "below 2^32 cells",
+             (long long)b.nx, (long long)b.ny, (long long)b.nz);
 }
 
 void launch_box_copy(const BoxCopy *boxes, int nbox, int64_t total, float *buf,

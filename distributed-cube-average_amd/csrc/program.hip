@@ -193,6 +193,7 @@ extern "C" int dcx_program_run(const dcx_program_args *args, float *out_pc,
               pb.ny = ub.ny = a1[1] - a0[1];
               pb.nz = ub.nz = a1[2] - a0[2];
               pb.off = ub.off = total;
+              check_multi_box(pb);
               total += pb.nx * pb.ny * pb.nz;
               pack[par].push_back(pb);
               unpack[par].push_back(ub);

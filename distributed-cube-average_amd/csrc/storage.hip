@@ -62,13 +62,20 @@ __global__ void k_decode_fields(const uint16_t *P, const uint16_t *D, float *pc,
   }
 }
 
-// Per-block (min, max, min positive, max positive) of a float array.
+// Per-block (min, max, min positive, max positive) of the sx x nrows logical
+// cells of a pitched array (a at x = 0, rows ys apart).  The row pitch, the
+// front pad and the tail are left out: no stencil reads them, and counting
+// their zeros would widen the code range of every array whose values are
+// all positive (and stop a constant array from coding exactly).
 __global__ void __launch_bounds__(256)
-    k_minmax(const float *a, int64_t n, float *out) {
+    k_minmax(const float *a, int64_t sx, int64_t ys, int64_t nrows,
+             float *out) {
   float lo = INFINITY, hi = -INFINITY, plo = INFINITY, phi = -INFINITY;
+  const int64_t n = sx * nrows;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = a[i];
+    const int64_t row = i / sx;
+    const float v = a[(i - row * sx) + row * ys];
     lo = fminf(lo, v);
     hi = fmaxf(hi, v);
     if (v > 0.0f) {
@@ -206,6 +213,11 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
   if (!fs_ok || !cs_ok) return -1;
   if (field_store == DCX_STORE_BF16 && coef_store == DCX_COEF_F32) return -1;
   if (coef_store == DCX_COEF_A16 && field_store != DCX_STORE_F32) return -1;
+  // A batch's launch tables and kernel follow the storage form the shots had
+  // when the batch was made; converting one shot would mix element sizes.
+  if (st->batch)
+    DCX_FAIL("dcx_set_storage on a batch shot: free the batch "
+             "(dcx_batch_free) before converting its shots");
   DCX_CHECK(hipStreamSynchronize(st->stream));
 
   // Bring the fields back to FP32 first if they are compact.
@@ -260,7 +272,8 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
     const int ncoded = coef_store == DCX_COEF_A16 ? V2PX : NCOEF;
     for (int k = 0; k < ncoded; ++k) {
       hipLaunchKernelGGL(k_minmax, dim3(kConvBlocks), dim3(256), 0, st->stream,
-                         whole(cs[k]), st->alloc_floats, d_mm);
+                         cs[k], st->sizes[0], st->ys,
+                         st->sizes[1] * st->sizes[2], d_mm);
       DCX_CHECK(hipGetLastError());
       DCX_CHECK(hipMemcpyAsync(mm.data(), d_mm, mm.size() * sizeof(float),
                                hipMemcpyDeviceToHost, st->stream));

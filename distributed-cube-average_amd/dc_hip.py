@@ -29,7 +29,7 @@ COEF_NAMES = ("ch1dxx", "ch1dyy", "ch1dzz", "ch1dxy", "ch1dyz", "ch1dxz",
 KERNEL_FAST = 0
 KERNEL_EXACT = 1
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
-COEF_F32, COEF_F16, COEF_U16 = 0, 1, 2
+COEF_F32, COEF_F16, COEF_U16, COEF_A16 = 0, 1, 2, 3
 
 _FP = C.POINTER(C.c_float)
 _SZ3 = C.c_size_t * 3

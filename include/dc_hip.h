@@ -158,7 +158,10 @@ void dcx_unpack_boxes(dc_device_data *data, const float *dev_buf, int nbox,
  * [recv_starts[3k..], recv_ends[3k..]) of the same arrays (a second launch).
  * Send and receive boxes must have equal shapes.  This is the on-device half
  * of a multi-GPU halo exchange with the transfer taken out; nbox = 0
- * removes it. */
+ * removes it.  As in the reference's worker (src/worker.c:466-511), the
+ * received halos never carry the next step's source: when the source cell
+ * lies in a send box, dcx_step / dcx_run add the next amplitude after the
+ * exchange (one small launch) instead of inside the stencil launch. */
 void dcx_set_halo_loopback(dc_device_data *data, int nbox,
                            const size_t *send_starts, const size_t *send_ends,
                            const size_t *recv_starts, const size_t *recv_ends);

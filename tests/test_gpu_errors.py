@@ -42,6 +42,16 @@ CASES = {
         "H.load().dc_device_extract_halo_face(d.ptr, H.fp(buf), "
         "H._SZ3(4, 4, 4), H._SZ3(8, 8, 8), H._SZ3(24, 22, 20), "
         "C.cast(C.c_void_p(4096), H._FP))", "not one of"),
+    # A batch's launch tables follow its shots' storage, kernel and halo
+    # settings as they were when it was made; changing one shot is refused.
+    "storage of a batch shot": (
+        "b = H.Batch([d]); d.set_storage(H.STORE_F16, H.COEF_F32)",
+        "batch shot"),
+    "kernel of a batch shot": (
+        "b = H.Batch([d]); d.set_kernel(H.KERNEL_EXACT)", "batch shot"),
+    "loopback on a batch shot": (
+        "b = H.Batch([d]); d.set_halo_loopback([((4, 4, 4), (8, 8, 8))], "
+        "[((4, 4, 4), (8, 8, 8))])", "batch shot"),
 }
 
 
