@@ -224,6 +224,9 @@ const State *state_of(const dc_device_data *d);
 // struct's four fields (returned as its device array); the box [start, end)
 // lies in the local grid (returned as a copy record of `arr`, offset 0).
 void check_sizes(const State *st, const size_t sizes[3]);
+// Resolve, once per device (the current one), the device address of the
+// staging array dcx_halo_face_noalloc uses.
+void noalloc_init(int device);
 float *field_of(const State *st, const float *token);
 BoxCopy box_of(const State *st, const size_t start[3], const size_t end[3],
                float *arr);

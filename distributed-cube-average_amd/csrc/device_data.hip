@@ -355,6 +355,7 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   st->magic = kMagic;
   st->device = select_device(process);
   DCX_CHECK(hipSetDevice(st->device));
+  noalloc_init(st->device);
   DCX_CHECK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
   st->own_stream = true;
   DCX_CHECK(hipEventCreate(&st->ev[0]));

@@ -118,6 +118,7 @@ SIGNATURES = {
     "dcx_scratch": (_FP, [_DD, C.c_size_t]),
     "dcx_set_halo_loopback": (None, [_DD, C.c_int] + [C.POINTER(C.c_size_t)] * 4),
     "dcx_copy_field": (C.c_size_t, [_DD, C.c_int, C.c_int]),
+    "dcx_halo_face_noalloc": (None, [_DD, _FP, _SZ3, _SZ3, _SZ3, _FP, C.c_int]),
     "dcx_batch_create": (C.c_void_p, [C.POINTER(_DD), C.c_int]),
     "dcx_batch_free": (None, [C.c_void_p]),
     "dcx_batch_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -292,6 +293,21 @@ class DeviceData:
         load().dc_device_insert_halo_face(
             self.ptr, fp(np.ascontiguousarray(buf, np.float32)), _SZ3(*start),
             _SZ3(*end), _SZ3(*self.sizes), self.field_token(field))
+
+    def extract_box_noalloc(self, start, end, field: str) -> np.ndarray:
+        """extract_box through dcx_halo_face_noalloc (no device allocation)."""
+        n = int(np.prod([e - s for s, e in zip(start, end)]))
+        buf = np.empty(n, np.float32)
+        load().dcx_halo_face_noalloc(self.ptr, fp(buf), _SZ3(*start),
+                                     _SZ3(*end), _SZ3(*self.sizes),
+                                     self.field_token(field), 0)
+        return buf
+
+    def insert_box_noalloc(self, buf: np.ndarray, start, end, field: str):
+        """insert_box through dcx_halo_face_noalloc (no device allocation)."""
+        load().dcx_halo_face_noalloc(
+            self.ptr, fp(np.ascontiguousarray(buf, np.float32)), _SZ3(*start),
+            _SZ3(*end), _SZ3(*self.sizes), self.field_token(field), 1)
 
     def get_results(self):
         """dc_device_data_get_results: returns copies of (pc, qc)."""

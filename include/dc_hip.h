@@ -166,6 +166,25 @@ void dcx_set_halo_loopback(dc_device_data *data, int nbox,
                            const size_t *send_starts, const size_t *send_ends,
                            const size_t *recv_starts, const size_t *recv_ends);
 
+/* dc_device_extract_halo_face (dir = 0: the box of `array` into the host
+ * `buffer`) and dc_device_insert_halo_face (dir = 1: `buffer` into the box)
+ * without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (tight exchange loops, harnesses that forbid
+ * hipMalloc; it waits for its copy, so it is not for graph capture).  The
+ * two reference
+ * entry points stage the packed box in a per-struct device buffer they
+ * allocate on first use and grow on demand; this one stages it in a fixed
+ * 16 MiB array that every device the library runs on holds until the process
+ * ends, so a box may hold at most 4,194,304 cells (larger boxes print an
+ * error and exit 1).  Same arguments, checks and element order as the
+ * reference entry points; returns when the copy is complete.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (from several host
+ * threads) race. */
+void dcx_halo_face_noalloc(dc_device_data *data, float *buffer,
+                           const size_t start_coords[3],
+                           const size_t end_coords[3], const size_t sizes[3],
+                           float *array, int dir);
+
 /* Copy the whole device allocation of field src over that of field dst
  * (0 pp, 1 pc, 2 qp, 3 qc; padding included) with the library's box-copy
  * kernel: a contiguous, aligned streaming copy of known size, used to
