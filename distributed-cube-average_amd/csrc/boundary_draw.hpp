@@ -121,6 +121,10 @@ DCX_HD inline float draw_fraction(int r) {
 
 DCX_HD inline void shell_velocity(const ShellGeom &s, const int g[3], float r,
                                   float *vp, float *vs) {
+#if defined(__clang__)
+  // The reference's separate roundings (no fused multiply-add) on the device.
+#pragma clang fp contract(off)
+#endif
   const int last = s.bord + s.absorb - 1, first = last + 1;
   const float frac = 1.0f / (float)s.absorb;
   const float maxP = 3000.0f;

@@ -224,6 +224,16 @@ const State *state_of(const dc_device_data *d);
 // struct's four fields (returned as its device array); the box [start, end)
 // lies in the local grid (returned as a copy record of `arr`, offset 0).
 void check_sizes(const State *st, const size_t sizes[3]);
+// The two halves of dc_device_data_init.  create_state: a struct for
+// process's local grid on its device (stream, pitched layout, the four fields
+// uploaded from process->pp ... or zero where NULL, through the packed
+// device buffer tmp of n floats or NULL, and zeroed coefficient arrays);
+// the caller then fills the coefficient arrays (coef_array(st, k), pitched,
+// dc_precomp_vars order) on the struct's stream; finish_state makes the
+// x-band copies, waits, and returns the caller's handle.
+State *create_state(const dc_process_t *process, float *tmp);
+float *coef_array(const State *st, int k);
+dc_device_data *finish_state(State *st);
 // Resolve, once per device (the current one), the device address of the
 // staging array dcx_halo_face_noalloc uses.
 void noalloc_init(int device);

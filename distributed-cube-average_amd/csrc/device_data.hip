@@ -348,6 +348,31 @@ extern "C" {
 
 dc_device_data *dc_device_data_init(dc_process_t *process) {
   if (process == nullptr) DCX_FAIL("null process");
+  const dc_precomp_vars &pv = process->precomp_vars;
+  const float *hc[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
+                            pv.ch1dyz, pv.ch1dxz, pv.v2px,   pv.v2pz,
+                            pv.v2sz,   pv.v2pn};
+  for (int k = 0; k < NCOEF; ++k)
+    if (hc[k] == nullptr) DCX_FAIL("precomp_vars array %d is NULL", k);
+  const size_t n = process->sizes[0] * process->sizes[1] * process->sizes[2];
+  float *tmp = nullptr;  // packed staging for the uploads
+  if (n > 0) DCX_CHECK(hipMalloc(&tmp, n * sizeof(float)));
+  State *st = create_state(process, tmp);
+  for (int k = 0; k < NCOEF; ++k) copy_in(st, coef_array(st, k), hc[k], tmp);
+  dc_device_data *d = finish_state(st);
+  DCX_CHECK(hipFree(tmp));
+  return d;
+}
+
+}  // extern "C"
+
+namespace dcx {
+
+float *coef_array(const State *st, int k) {
+  return st->coef_alloc + (size_t)k * st->alloc_floats + kPad;
+}
+
+State *create_state(const dc_process_t *process, float *tmp) {
   State *st = static_cast<State *>(std::calloc(1, sizeof(State)));
   if (st == nullptr)
     DCX_FAIL("OOM: could not allocate memory for device_data in "
@@ -381,26 +406,20 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
   const size_t abytes = (size_t)st->alloc_floats * sizeof(float);
 
   float *host_fields[4] = {process->pp, process->pc, process->qp, process->qc};
-  float *tmp = nullptr;  // packed staging for the uploads below
-  DCX_CHECK(hipMalloc(&tmp, (size_t)st->n * sizeof(float)));
   for (int k = 0; k < 4; ++k) {
     DCX_CHECK(hipMalloc(&st->field_alloc[k], abytes));
     DCX_CHECK(hipMemsetAsync(st->field_alloc[k], 0, abytes, st->stream));
     st->field_mem[k] = st->field_alloc[k] + kPad;
     if (host_fields[k]) copy_in(st, st->field_mem[k], host_fields[k], tmp);
   }
-  const dc_precomp_vars &pv = process->precomp_vars;
-  const float *hc[NCOEF] = {pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy,
-                            pv.ch1dyz, pv.ch1dxz, pv.v2px,   pv.v2pz,
-                            pv.v2sz,   pv.v2pn};
   DCX_CHECK(hipMalloc(&st->coef_alloc, abytes * NCOEF));
   DCX_CHECK(hipMemsetAsync(st->coef_alloc, 0, abytes * NCOEF, st->stream));
+  return st;
+}
+
+dc_device_data *finish_state(State *st) {
   float *dc[NCOEF];
-  for (int k = 0; k < NCOEF; ++k) {
-    if (hc[k] == nullptr) DCX_FAIL("precomp_vars array %d is NULL", k);
-    dc[k] = st->coef_alloc + (size_t)k * st->alloc_floats + kPad;
-    copy_in(st, dc[k], hc[k], tmp);
-  }
+  for (int k = 0; k < NCOEF; ++k) dc[k] = coef_array(st, k);
   // Dense copies of the coefficients over the two x bands (one 2-D copy of
   // 16-byte rows per array and band).
   if (st->sizes[0] >= 4 * STENCIL) {
@@ -417,7 +436,6 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
     }
   }
   DCX_CHECK(hipStreamSynchronize(st->stream));
-  DCX_CHECK(hipFree(tmp));
 
   dc_device_data &d = st->abi;
   d.pp = st->field_mem[0];
@@ -430,6 +448,10 @@ dc_device_data *dc_device_data_init(dc_process_t *process) {
                                    dc[5], dc[6], dc[7], dc[8], dc[9]};
   return &d;
 }
+
+}  // namespace dcx
+
+extern "C" {
 
 void dc_device_data_free(dc_device_data *data) {
   if (data == nullptr) return;
@@ -613,6 +635,13 @@ void dcx_download(dc_device_data *data, int which, float *host) {
     return;
   }
   copy_out(st, host, field_index(st, which));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+}
+
+void dcx_download_coef(dc_device_data *data, int k, float *host) {
+  State *st = state_of(data);
+  if (k < 0 || k >= NCOEF) DCX_FAIL("coefficient index %d not in 0..9", k);
+  copy_out(st, host, coef_array(st, k));
   DCX_CHECK(hipStreamSynchronize(st->stream));
 }
 

@@ -58,23 +58,10 @@ extern "C" int dcx_program_run(const dcx_program_args *args, float *out_pc,
   const unsigned iterations =
       (unsigned)std::ceil(args->time_max / args->dt);
 
-  // Global model once (every rank's share is a slice of it: the default
-  // model is uniform, the boundary draws depend only on the global cell, and
-  // the precompute is per cell).
-  const int64_t gn = g[0] * g[1] * g[2];
-  std::vector<float> vpz(gn), vsv(gn), eps(gn), del(gn), phi(gn), th(gn);
-  dcx_default_model((size_t)gn, vpz.data(), vsv.data(), eps.data(), del.data(),
-                    phi.data(), th.data());
-  unsigned int seed = 0;
-  dcx_random_boundary((int)g[0], (int)g[1], (int)g[2], (int)g[0], (int)g[1],
-                      (int)g[2], 0, 0, 0, (int)prob[0], (int)prob[1],
-                      (int)prob[2], STENCIL, (int)ab, vpz.data(), vsv.data(),
-                      &seed);
-  std::vector<float> coef((size_t)gn * 10);
-  float *cp[10];
-  for (int k = 0; k < 10; ++k) cp[k] = coef.data() + (size_t)k * gn;
-  dcx_precompute((size_t)gn, vpz.data(), vsv.data(), eps.data(), del.data(),
-                 phi.data(), th.data(), cp);
+  if (g[0] > INT32_MAX || g[1] > INT32_MAX || g[2] > INT32_MAX)
+    DCX_FAIL("global grid too large");
+  const int gsz[3] = {(int)g[0], (int)g[1], (int)g[2]};
+  const int psz[3] = {(int)prob[0], (int)prob[1], (int)prob[2]};
   // Source at the global centre (src/coordinator.c:11-16).
   const int64_t sc[3] = {g[0] / 2, g[1] / 2, g[2] / 2};
 
@@ -105,18 +92,6 @@ extern "C" int dcx_program_run(const dcx_program_args *args, float *out_pc,
                                   (sc[1] - s.start[1]) * s.size[0] +
                                   (sc[2] - s.start[2]) * s.size[0] * s.size[1])
                     : -1;
-        // Slice the model into this subdomain's arrays.
-        const int64_t ln = s.size[0] * s.size[1] * s.size[2];
-        std::vector<float> lc((size_t)ln * 10);
-        for (int64_t z = 0; z < s.size[2]; ++z)
-          for (int64_t y = 0; y < s.size[1]; ++y) {
-            const int64_t gi = s.start[0] + (s.start[1] + y) * g[0] +
-                               (s.start[2] + z) * g[0] * g[1];
-            const int64_t li = (y + z * s.size[1]) * s.size[0];
-            for (int k = 0; k < 10; ++k)
-              std::memcpy(lc.data() + (size_t)k * ln + li, cp[k] + gi,
-                          (size_t)s.size[0] * sizeof(float));
-          }
         dc_process_t p{};
         for (int k = 0; k < 3; ++k) {
           p.coordinates[k] = c[k];
@@ -129,12 +104,12 @@ extern "C" int dcx_program_run(const dcx_program_args *args, float *out_pc,
         p.dy = args->dy;
         p.dz = args->dz;
         p.dt = args->dt;
-        p.precomp_vars = dc_precomp_vars{
-            lc.data(),          lc.data() + ln,     lc.data() + 2 * ln,
-            lc.data() + 3 * ln, lc.data() + 4 * ln, lc.data() + 5 * ln,
-            lc.data() + 6 * ln, lc.data() + 7 * ln, lc.data() + 8 * ln,
-            lc.data() + 9 * ln};
-        s.d = dc_device_data_init(&p);  // fields start at zero
+        // The rank's model, built on the device as the reference's worker
+        // builds it on the host (src/worker.c:49-156); every rank's shell
+        // walk starts from seed 0 (src/worker.c:93).  Fields start at zero.
+        const int st0[3] = {(int)s.start[0], (int)s.start[1], (int)s.start[2]};
+        unsigned int seed = 0;
+        s.d = dcx_device_data_init_model(&p, gsz, st0, psz, (int)ab, &seed);
         State *st = state_of(s.d);
         adopt_stream(st, stream);
         st->kernel = args->kernel;
@@ -245,6 +220,7 @@ extern "C" int dcx_program_run(const dcx_program_args *args, float *out_pc,
 
   // Gather the compute regions into the global layout.
   if (out_pc || out_qc) {
+    const int64_t gn = g[0] * g[1] * g[2];
     if (out_pc) std::memset(out_pc, 0, (size_t)gn * sizeof(float));
     if (out_qc) std::memset(out_qc, 0, (size_t)gn * sizeof(float));
     for (const Sub &sb : subs) {

@@ -155,6 +155,9 @@ SIGNATURES = {
                                     C.c_size_t, _FP, _FP]),
     "dcx_default_model": (None, [C.c_size_t] + [_FP] * 6),
     "dcx_precompute": (None, [C.c_size_t] + [_FP] * 6 + [_FP * 10]),
+    "dcx_device_data_init_model": (_DD, [C.POINTER(Process), _I3, _I3, _I3,
+                                         C.c_int, C.POINTER(C.c_uint)]),
+    "dcx_download_coef": (None, [_DD, C.c_int, _FP]),
     "dcx_random_boundary": (None, [C.c_int] * 14 + [_FP, _FP,
                                                     C.POINTER(C.c_uint)]),
     "dcx_ricker": (C.c_float, [C.c_float, C.c_int, C.c_double]),
@@ -260,6 +263,37 @@ class DeviceData:
         self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
         self.ptr = lib.dc_device_data_init(C.byref(p))
         self.set_kernel(kernel)
+
+    @classmethod
+    def from_default_model(cls, local_sizes, global_sizes, start, problem,
+                           absorption, seed=0, dx=20.0, dy=20.0, dz=20.0,
+                           dt=1e-3, kernel=KERNEL_FAST):
+        """dcx_device_data_init_model: the reference worker's model (default
+        TTI model, random-velocity shell, coefficients) built on the device
+        for the local box `local_sizes` (x, y, z) at global cell `start`.
+        Returns (DeviceData, seed after the shell's draws); fields zero."""
+        self = cls.__new__(cls)
+        p = Process()
+        p.sizes = _SZ3(*local_sizes)
+        p.dx, p.dy, p.dz, p.dt = dx, dy, dz, dt
+        p.source_index = -1
+        p.topology = _I3(1, 1, 1)
+        self._proc, self._coef, self._fields = p, None, None
+        self.sizes = tuple(int(v) for v in local_sizes)
+        self.shape = self.sizes[::-1]
+        self.dx, self.dy, self.dz, self.dt = dx, dy, dz, dt
+        s = C.c_uint(seed)
+        self.ptr = load().dcx_device_data_init_model(
+            C.byref(p), _I3(*global_sizes), _I3(*start), _I3(*problem),
+            int(absorption), C.byref(s))
+        self.set_kernel(kernel)
+        return self, s.value
+
+    def download_coef(self, k: int) -> np.ndarray:
+        """Coefficient array k (dc_precomp_vars order) as the device holds it."""
+        out = np.empty(self.shape, np.float32)
+        load().dcx_download_coef(self.ptr, k, fp(out))
+        return out
 
     # -- reference ABI ------------------------------------------------------
     def propagate(self, start, end, dx=None, dy=None, dz=None, dt=None):

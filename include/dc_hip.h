@@ -393,6 +393,34 @@ void dcx_precompute(size_t n, const float *vpz, const float *vsv,
                     const float *epsilon, const float *delta, const float *phi,
                     const float *theta, float *const out[10]);
 
+/* dc_device_data_init with the reference worker's model setup
+ * (src/worker.c:49-156) computed on the device instead of passed in: the
+ * default TTI model (src/precomp.c:156-169) in every local cell, the
+ * random-velocity shell (src/boundary.c:7-104) over the rank's box of the
+ * global grid, and the ten coefficient arrays (src/precomp.c:85-105), one
+ * thread per local cell, each shell cell finding its own rand_r draw in
+ * closed form instead of the reference's walk over the whole global grid.
+ * The default model's trigonometry is one (theta, phi) pair, done in FP64 on
+ * the host.  process->sizes gives the local grid (halo included) and
+ * process->pp/pc/qp/qc the initial fields (NULL: zero); precomp_vars and
+ * anisotropy_vars are not read.  global_sizes, start_coords (global
+ * coordinates of local cell 0) and problem_sizes are the arguments the
+ * reference worker passes to randomVelocityBoundaryPartition with
+ * bord = STENCIL (src/worker.c:94-103); *seed is the generator state before
+ * the walk (0 in the reference) and is left where the walk leaves it.  The
+ * coefficient arrays equal, bit for bit, those dc_device_data_init would
+ * upload from the reference's host setup. */
+dc_device_data *dcx_device_data_init_model(dc_process_t *process,
+                                           const int global_sizes[3],
+                                           const int start_coords[3],
+                                           const int problem_sizes[3],
+                                           int absorption, unsigned int *seed);
+
+/* Copy coefficient array k (dc_precomp_vars order, 0 = ch1dxx ... 9 = v2pn,
+ * FP32) of the struct to host memory (sizes[0]*sizes[1]*sizes[2] floats, x
+ * fastest). */
+void dcx_download_coef(dc_device_data *data, int k, float *host);
+
 /* src/boundary.c:7-104: random-velocity boundary over one local box. */
 void dcx_random_boundary(int local_sx, int local_sy, int local_sz,
                          int global_sx, int global_sy, int global_sz,
