@@ -124,6 +124,26 @@ void launch_box_copy1(const BoxCopy &box, float *buf, int dir, hipStream_t s);
 // coordinates are 32-bit, so each box must hold fewer than 2^32 cells.
 void check_multi_box(const BoxCopy &b);
 
+// dc_propagate boxes one struct queues before launching them together (the
+// reference's worker makes seven calls per step, src/worker.c:321-402).
+constexpr int kMaxPending = 8;
+
+// Several boxes of one array set in one fast-kernel launch: tile prefix sums,
+// and per box its bounds, tile grid, z chunk and coefficient arrays (the
+// pitched arrays, or an x band's dense copy).  Passed by value.
+struct MultiBox {
+  int n;
+  int64_t prefix[kMaxPending + 1];
+  Box box[kMaxPending];
+  int gx[kMaxPending], gy[kMaxPending], zchunk[kMaxPending];
+  int cx0[kMaxPending], cw[kMaxPending];
+  int64_t czs[kMaxPending];
+  const void *cf[kMaxPending][NCOEF];
+};
+// One launch of the 32 x 16 fast kernel over the boxes of m, which share
+// a's arrays, spacings and dt (a.box and the per-box fields are ignored).
+void launch_fast_multi(const StepArgs &a, const MultiBox &m, hipStream_t s);
+
 // Library state behind a dc_device_data pointer.  `abi` must stay first: the
 // caller holds &state->abi.
 struct State {
@@ -196,6 +216,12 @@ struct State {
   uint32_t *sample_cells;
   // The batch the struct belongs to (dcx_batch_create), if any.
   const void *batch;
+  // dc_propagate calls not yet launched: their boxes and the launch
+  // arguments they share (arrays, spacings, dt).  Every other entry point
+  // launches them first (state_of), as one launch (flush_pending).
+  int npend;
+  Box pend[kMaxPending];
+  StepArgs pend_args;
 };
 // Count of structs a network is attached to (it cannot be freed before).
 void mlp_attach(const void *mlp, int delta);
@@ -216,8 +242,13 @@ __host__ __device__ inline int64_t tile_origin_x(int64_t x0) {
 // Device offset of the caller's linear index x + y*sx + z*sx*sy.
 int64_t device_index(const State *st, int64_t logical);
 
+// The struct behind a caller's handle (fails for a foreign pointer), with its
+// queued dc_propagate boxes launched first; state_peek leaves them queued.
 State *state_of(dc_device_data *d);
 const State *state_of(const dc_device_data *d);
+State *state_peek(dc_device_data *d);
+// Launch the queued dc_propagate boxes of st (none: no-op).
+void flush_pending(State *st);
 
 // Host checks of the reference ABI's box calls (each fails with a message and
 // exit 1): the caller's sizes equal the struct's; the token names one of the

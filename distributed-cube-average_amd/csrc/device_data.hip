@@ -1,6 +1,7 @@
 // The reference's backend ABI (include/propagate.h, include/device_data.h) on
 // one MI355X, plus the library's own dcx_* entry points for device-resident
 // time loops.  All work of one dc_device_data is ordered on one HIP stream.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -9,11 +10,17 @@
 
 namespace dcx {
 
-State *state_of(dc_device_data *d) {
+State *state_peek(dc_device_data *d) {
   if (d == nullptr) DCX_FAIL("null dc_device_data");
   State *st = reinterpret_cast<State *>(d);
   if (st->magic != kMagic)
     DCX_FAIL("dc_device_data was not created by dc_device_data_init");
+  return st;
+}
+
+State *state_of(dc_device_data *d) {
+  State *st = state_peek(d);
+  flush_pending(st);
   return st;
 }
 
@@ -111,15 +118,10 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   return a;
 }
 
-void launch_stencil(const State *st, const StepArgs &a) {
-  if (st->kernel == DCX_KERNEL_EXACT) {
-    if (st->fstore != DCX_STORE_F32 || st->cstore != DCX_COEF_F32)
-      DCX_FAIL("the exact kernel runs on FP32 storage only");
-    launch_exact(a, st->stream);
-    return;
-  }
-  // A box inside one of the two 4-cell x bands (the reference worker's x
-  // boundary slabs) reads its FP32 coefficients from the band's dense copy.
+// The fast kernel's arguments for a's box: a box inside one of the two
+// 4-cell x bands (the reference worker's x boundary slabs) reads its FP32
+// coefficients from the band's dense copy.
+static StepArgs with_band(const State *st, const StepArgs &a) {
   if (st->xband && a.cstore == DCX_COEF_F32) {
     const int64_t band = STENCIL * st->sizes[1] * st->sizes[2];
     for (int side = 0; side < 2; ++side) {
@@ -131,12 +133,92 @@ void launch_stencil(const State *st, const StepArgs &a) {
         b.cx0 = (int)bx0;
         b.cw = STENCIL;
         b.czs = STENCIL * st->sizes[1];
-        launch_fast(b, st->tile_x, st->tile_y, st->stream);
-        return;
+        return b;
       }
     }
   }
-  launch_fast(a, st->tile_x, st->tile_y, st->stream);
+  return a;
+}
+
+void launch_stencil(const State *st, const StepArgs &a) {
+  if (st->kernel == DCX_KERNEL_EXACT) {
+    if (st->fstore != DCX_STORE_F32 || st->cstore != DCX_COEF_F32)
+      DCX_FAIL("the exact kernel runs on FP32 storage only");
+    launch_exact(a, st->stream);
+    return;
+  }
+  launch_fast(with_band(st, a), st->tile_x, st->tile_y, st->stream);
+}
+
+static int64_t box_volume(const Box &b) {
+  return (b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
+}
+
+static bool boxes_overlap(const Box &a, const Box &b) {
+  return a.x0 < b.x1 && b.x0 < a.x1 && a.y0 < b.y1 && b.y0 < a.y1 &&
+         a.z0 < b.z1 && b.z0 < a.z1;
+}
+
+// Launch the queued dc_propagate boxes.  They are pairwise disjoint (a box
+// that overlaps a queued one flushes the queue first) and every cell's update
+// reads only the current fields, so their order does not matter.  When the
+// boxes tile their bounding box exactly -- the reference worker's six slabs
+// and interior cover the compute region -- they run as one launch over it;
+// otherwise the fast kernel takes them all in one multi-box launch (the
+// exact kernel, or forms without one, run them one by one).  The results are
+// those of the separate calls, bit for bit.
+void flush_pending(State *st) {
+  const int n = st->npend;
+  if (n == 0) return;
+  st->npend = 0;
+  StepArgs a = st->pend_args;
+  Box hull = st->pend[0];
+  int64_t vol = 0;
+  for (int k = 0; k < n; ++k) {
+    const Box &b = st->pend[k];
+    hull.x0 = std::min(hull.x0, b.x0);
+    hull.y0 = std::min(hull.y0, b.y0);
+    hull.z0 = std::min(hull.z0, b.z0);
+    hull.x1 = std::max(hull.x1, b.x1);
+    hull.y1 = std::max(hull.y1, b.y1);
+    hull.z1 = std::max(hull.z1, b.z1);
+    vol += box_volume(b);
+  }
+  if (n == 1 || vol == box_volume(hull)) {
+    a.box = n == 1 ? st->pend[0] : hull;
+    if (n > 1 && st->zchunk == 0) a.zchunk = auto_zchunk(st, a.box);
+    launch_stencil(st, a);
+    return;
+  }
+  const bool multi = st->kernel == DCX_KERNEL_FAST && st->tile_x == 32 &&
+                     st->tile_y == 16 && a.fstore == DCX_STORE_F32 &&
+                     a.cstore == DCX_COEF_F32;
+  if (!multi) {
+    for (int k = 0; k < n; ++k) {
+      a.box = st->pend[k];
+      if (st->zchunk == 0) a.zchunk = auto_zchunk(st, a.box);
+      launch_stencil(st, a);
+    }
+    return;
+  }
+  MultiBox m{};
+  m.n = n;
+  for (int k = 0; k < n; ++k) {
+    StepArgs b = a;
+    b.box = st->pend[k];
+    b.zchunk = st->zchunk > 0 ? st->zchunk : auto_zchunk(st, b.box);
+    b = with_band(st, b);
+    m.box[k] = b.box;
+    m.zchunk[k] = b.zchunk;
+    m.gx[k] = (int)((b.box.x1 - tile_origin_x(b.box.x0) + 31) / 32);
+    m.gy[k] = (int)((b.box.y1 - b.box.y0 + 15) / 16);
+    m.cx0[k] = b.cx0;
+    m.cw[k] = b.cw;
+    m.czs[k] = b.czs;
+    for (int c = 0; c < NCOEF; ++c) m.cf[k][c] = b.cf[c];
+    m.prefix[k + 1] = m.prefix[k] + fast_tiles(b, 32, 16);
+  }
+  launch_fast_multi(a, m, st->stream);
 }
 
 void adopt_stream(State *st, hipStream_t s) {
@@ -491,7 +573,8 @@ void dc_device_data_get_results(dc_process_t *process, dc_device_data *data) {
 }
 
 void dc_device_swap_arrays(dc_device_data *data) {
-  state_of(data);
+  // Host pointers only: queued boxes keep the arrays they were queued with.
+  state_peek(data);
   float *t = data->pp;
   data->pp = data->pc;
   data->pc = t;
@@ -558,7 +641,7 @@ void dc_propagate(const size_t start_coords[DIMENSIONS],
                   const float dt) {
   (void)process_coordinates;
   (void)topology;
-  State *st = state_of(data);
+  State *st = state_peek(data);  // queued boxes stay queued
   check_sizes(st, sizes);
   if (st->fstore != DCX_STORE_F32)
     DCX_FAIL("dc_propagate runs on FP32 field storage (dcx_set_storage)");
@@ -573,7 +656,23 @@ void dc_propagate(const size_t start_coords[DIMENSIONS],
   const Box box{(int64_t)start_coords[0], (int64_t)start_coords[1],
                 (int64_t)start_coords[2], (int64_t)end_coords[0],
                 (int64_t)end_coords[1], (int64_t)end_coords[2]};
-  launch_stencil(st, make_step_args(st, box, dx, dy, dz, dt));
+  // Queue the box; the next entry point that needs the fields launches the
+  // queue (flush_pending).  A box that cannot join the queue -- other
+  // arrays (after a swap), other spacings or dt, an overlap with a queued
+  // box, a full queue -- launches the queue first.
+  const StepArgs a = make_step_args(st, box, dx, dy, dz, dt);
+  if (st->npend > 0) {
+    const StepArgs &q = st->pend_args;
+    bool join = st->npend < kMaxPending && a.pp == q.pp && a.pc == q.pc &&
+                a.qp == q.qp && a.qc == q.qc && a.dxx == q.dxx &&
+                a.dyy == q.dyy && a.dzz == q.dzz && a.dxy == q.dxy &&
+                a.dxz == q.dxz && a.dyz == q.dyz && a.dt == q.dt;
+    for (int k = 0; join && k < st->npend; ++k)
+      join = !boxes_overlap(st->pend[k], box);
+    if (!join) flush_pending(st);
+  }
+  if (st->npend == 0) st->pend_args = a;
+  st->pend[st->npend++] = box;
 }
 
 int dcx_set_kernel(dc_device_data *data, int kernel) {

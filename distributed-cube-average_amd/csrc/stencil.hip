@@ -653,6 +653,48 @@ __global__ void __launch_bounds__(TX *TY)
   }
 }
 
+// Several boxes of one array set in one launch (queued dc_propagate calls):
+// workgroup b finds its box by the tile prefix sums (uniform per workgroup)
+// and runs that box's tile with the box's own bounds, tile grid, z chunk and
+// coefficient arrays.
+template <int TX, int TY, int PF, int FS, int CS>
+__global__ void __launch_bounds__(TX *TY)
+    k_fast_multi(const StepArgs a, const MultiBox m) {
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < m.n && m.prefix[k + 1] <= b) ++k;
+  StepArgs s = a;
+  s.box = m.box[k];
+  s.gx = m.gx[k];
+  s.gy = m.gy[k];
+  s.zchunk = m.zchunk[k];
+  s.cx0 = m.cx0[k];
+  s.cw = m.cw[k];
+  s.czs = m.czs[k];
+#pragma unroll
+  for (int c = 0; c < NCOEF; ++c) s.cf[c] = m.cf[k][c];
+  fast_tile<TX, TY, PF, FS, CS>(s, b - (int)m.prefix[k], a.src_amp);
+}
+
+void launch_fast_multi(const StepArgs &a, const MultiBox &m, hipStream_t s) {
+  if (m.n <= 0 || m.prefix[m.n] <= 0) return;
+  if (a.fstore != DCX_STORE_F32 || a.cstore != DCX_COEF_F32)
+    DCX_FAIL("multi-box launches run FP32 storage only");
+  if (m.prefix[m.n] >= (int64_t(1) << 31)) DCX_FAIL("too many tiles");
+  for (int k = 0; k < m.n; ++k)
+    if ((m.box[k].z1 + 4) * a.zs * 4 > (int64_t(1) << 32) - 1)
+      DCX_FAIL("fast kernel needs arrays below 4 GiB");
+  const dim3 grid((unsigned)m.prefix[m.n]), block(32 * 16);
+  constexpr int F = DCX_STORE_F32, C = DCX_COEF_F32;
+  if (a.deep >= 2)
+    hipLaunchKernelGGL((k_fast_multi<32, 16, 3, F, C>), grid, block, 0, s, a, m);
+  else if (a.deep == 1)
+    hipLaunchKernelGGL((k_fast_multi<32, 16, 2, F, C>), grid, block, 0, s, a, m);
+  else
+    hipLaunchKernelGGL((k_fast_multi<32, 16, 1, F, C>), grid, block, 0, s, a, m);
+  DCX_CHECK(hipGetLastError());
+}
+
 // Tile shapes with a kernel instance: 512 threads (32x16, 64x8) or 1024
 // threads (32x32, 64x16; at 1024 threads a lane has 128 registers, and these
 // spill).
