@@ -187,9 +187,9 @@ def cpu_baseline(w, fields, steps, log_prefix="[bench]"):
 
 
 # Storage forms measured beside the plain FP32 step: (name, field, coef).
-FORMS = [("fp32", 0, 0), ("fp32_u16coef", 0, 2), ("fp32_a16coef", 0, 3),
-         ("fp16_f32coef", 1, 0), ("fp16", 1, 1), ("fp16_u16coef", 1, 2),
-         ("bf16_u16coef", 2, 2)]
+FORMS = [("fp32", 0, 0), ("fp32_n16coef", 0, 4), ("fp32_u16coef", 0, 2),
+         ("fp32_a16coef", 0, 3), ("fp16_f32coef", 1, 0), ("fp16", 1, 1),
+         ("fp16_u16coef", 1, 2), ("bf16_u16coef", 2, 2)]
 # Half field storage scale 2^8: the field peaks near 40 (true units) at the
 # source cell and half's largest finite value is 65504.
 FIELD_SCALE_LOG2 = 8
@@ -273,7 +273,7 @@ def precision_study(H, VM, steps_list, log_prefix="[bench]"):
     return res
 
 
-BATCH_FORMS = [("fp32", 0, 0), ("fp32_a16coef", 0, 3), ("fp16", 1, 1)]
+BATCH_FORMS = [("fp32", 0, 0), ("fp32_n16coef", 0, 4), ("fp16", 1, 1)]
 
 
 def batch_study(H, VM, args):

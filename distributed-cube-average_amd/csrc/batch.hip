@@ -185,7 +185,8 @@ struct Plan {
         else
           st->pcur ^= 1;
       }
-      if ((int64_t)(box.z1 + 4) * st->zs * 4 > (int64_t(1) << 32) - 1)
+      const int64_t ebytes = st->cstore == DCX_COEF_N16 ? 8 : 4;
+      if ((int64_t)(box.z1 + 4) * st->zs * ebytes > (int64_t(1) << 32) - 1)
         DCX_FAIL("shot %d arrays exceed 4 GiB", k);
       prefix[k + 1] = prefix[k] + fast_tiles(args[0][k], 32, 16);
     }

@@ -280,6 +280,17 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
 void launch_stencil(const State *st, const StepArgs &a);
 
 // Compact storage (storage.hip).
+// DCX_COEF_N16's code arrays in the C16 allocation (12 B per cell): a dword
+// of (a, b) codes, then a qword of (c, rx, rn, rs) codes, each laid out as
+// the fields (element x = 0 of row 0 at index 0, kPad elements in).
+inline uint32_t *n16_ab(const State *st) {
+  return reinterpret_cast<uint32_t *>(st->C16 - kPad) + kPad;
+}
+inline uint64_t *n16_cr(const State *st) {
+  return reinterpret_cast<uint64_t *>(
+             reinterpret_cast<char *>(st->C16 - kPad) + 4 * st->alloc_floats) +
+         kPad;
+}
 void storage_free(State *st);
 void storage_download(const State *st, int which, float *host);
 void storage_add_source(State *st, int64_t dev_index, float amp);

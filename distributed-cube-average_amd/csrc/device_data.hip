@@ -115,6 +115,15 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
     a.c_lo[k] = st->c_lo[k];
     a.c_st[k] = st->c_st[k];
   }
+  if (st->cstore == DCX_COEF_N16) {
+    // The axis form's dword (a, b) and qword (c, rx, rn, rs) code arrays
+    // share the C16 allocation; v2pz stays the FP32 array.
+    const void *ab = n16_ab(st), *cr = n16_cr(st);
+    for (int k = 0; k < NCOEF; ++k) a.cf[k] = nullptr;
+    a.cf[CXX] = a.cf[CZZ] = ab;
+    a.cf[CXZ] = a.cf[CYZ] = cr;
+    a.cf[V2PZ] = a.coef.c[V2PZ];
+  }
   return a;
 }
 

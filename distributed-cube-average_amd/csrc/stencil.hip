@@ -289,6 +289,35 @@ template <> struct CoefIO<DCX_COEF_A16> {
   }
 };
 
+// TTI axis form: per cell a dword AB = (a, b) codes, a qword CR = (c, rx, rn,
+// rs) codes and the FP32 v2pz array.  The in-plane part of H1 needs a and b
+// of the staged plane (raw slot cxx = AB); the output plane needs AB (raw
+// slot czz), CR (slots cxz, cyz) and v2pz (slot vpz).  Offsets are formed
+// for 4-byte elements and doubled for CR (every array below 4 GiB, checked
+// at launch).  Decode parameters: c_lo/c_st[0..5] for a, b, c, rx, rn, rs.
+template <> struct CoefIO<DCX_COEF_N16> {
+  static constexpr uint32_t kBytes = 4;
+  static __device__ __forceinline__ uint32_t ldr(const StepArgs &a, int k,
+                                                 uint32_t o) {
+    switch (k) {
+      case CXX: return ldb<uint32_t>(a.cf[CXX], o);
+      case CZZ: return ldb<uint32_t>(a.cf[CZZ], o);
+      case CXZ: return ldb<uint32_t>(a.cf[CXZ], 2u * o);
+      case CYZ: return ldb<uint32_t>(a.cf[CXZ], 2u * o + 4u);
+      case V2PZ: return ldb<uint32_t>(a.cf[V2PZ], o);
+      default: return 0u;
+    }
+  }
+  static __device__ __forceinline__ float lo(const StepArgs &a, int k,
+                                             uint32_t r) {
+    return fmaf((float)(r & 0xffffu), a.c_st[k], a.c_lo[k]);
+  }
+  static __device__ __forceinline__ float hi(const StepArgs &a, int k,
+                                             uint32_t r) {
+    return fmaf((float)(r >> 16), a.c_st[k], a.c_lo[k]);
+  }
+};
+
 // Per-step global inputs, raw: coefficients of the staged plane (in-plane
 // part of H1) and of the output plane, and the output plane's old state
 // (previous level for leapfrog storage, time difference otherwise).
@@ -550,10 +579,29 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
                            W_L3 * (RS(yq, 3) - RS(yq, 5)) +
                            W_L4 * (dyq - yq[O])) * a.dyz;
 #undef RS
-        const float czz = Cf::dec(a, CZZ, L.czz), cxz = Cf::dec(a, CXZ, L.cxz),
-                    cyz = Cf::dec(a, CYZ, L.cyz), vpx = Cf::dec(a, V2PX, L.vpx),
-                    vpz = Cf::dec(a, V2PZ, L.vpz), vsz = Cf::dec(a, V2SZ, L.vsz),
-                    vpn = Cf::dec(a, V2PN, L.vpn);
+        float czz, cxz, cyz, vpx, vpz, vsz, vpn;
+        if constexpr (CS == DCX_COEF_N16) {
+          // n = (na, nb, nc): czz = nc^2, cxz = 2 na nc, cyz = 2 nb nc;
+          // v2px, v2pn, v2sz = v2pz times the three ratios.
+          const float na = Cf::lo(a, 0, L.czz), nb = Cf::hi(a, 1, L.czz);
+          const float nc = Cf::lo(a, 2, L.cxz);
+          const float c2 = nc + nc;
+          czz = nc * nc;
+          cxz = c2 * na;
+          cyz = c2 * nb;
+          vpz = __uint_as_float(L.vpz);
+          vpx = vpz * Cf::hi(a, 3, L.cxz);
+          vpn = vpz * Cf::lo(a, 4, L.cyz);
+          vsz = vpz * Cf::hi(a, 5, L.cyz);
+        } else {
+          czz = Cf::dec(a, CZZ, L.czz);
+          cxz = Cf::dec(a, CXZ, L.cxz);
+          cyz = Cf::dec(a, CYZ, L.cyz);
+          vpx = Cf::dec(a, V2PX, L.vpx);
+          vpz = Cf::dec(a, V2PZ, L.vpz);
+          vsz = Cf::dec(a, V2SZ, L.vsz);
+          vpn = Cf::dec(a, V2PN, L.vpn);
+        }
         const float opp = F::dec(L.opp), oqp = F::dec(L.oqp);
         const int D = u & 3;
         const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
@@ -604,8 +652,17 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       yp[u] = dyp;
       xq[u] = dxq;
       yq[u] = dyq;
-      const float cxx = Cf::dec(a, CXX, L.cxx), cyy = Cf::dec(a, CYY, L.cyy),
-                  cxy = Cf::dec(a, CXY, L.cxy);
+      float cxx, cyy, cxy;
+      if constexpr (CS == DCX_COEF_N16) {
+        const float na = Cf::lo(a, 0, L.cxx), nb = Cf::hi(a, 1, L.cxx);
+        cxx = na * na;
+        cyy = nb * nb;
+        cxy = (na + na) * nb;
+      } else {
+        cxx = Cf::dec(a, CXX, L.cxx);
+        cyy = Cf::dec(a, CYY, L.cyy);
+        cxy = Cf::dec(a, CXY, L.cxy);
+      }
       Sp[u & 3] = pxx + pyy;
       Hp[u & 3] = cxx * pxx + cyy * pyy + cxy * pxy;
       Sq[u & 3] = qxx + qyy;
@@ -729,6 +786,7 @@ static void launch_fast_tile(const StepArgs &b, dim3 grid, dim3 block,
   DCX_FORM(DCX_STORE_F32, DCX_COEF_F16)
   DCX_FORM(DCX_STORE_F32, DCX_COEF_U16)
   DCX_FORM(DCX_STORE_F32, DCX_COEF_A16)
+  DCX_FORM(DCX_STORE_F32, DCX_COEF_N16)
   DCX_FORM(DCX_STORE_F16, DCX_COEF_F32)
   DCX_FORM(DCX_STORE_F16, DCX_COEF_F16)
   DCX_FORM(DCX_STORE_F16, DCX_COEF_U16)
@@ -786,6 +844,7 @@ void launch_fast_batch(const StepArgs *shots, const int64_t *prefix,
   DCX_BFORM(DCX_STORE_F32, DCX_COEF_F32)
   DCX_BFORM(DCX_STORE_F32, DCX_COEF_U16)
   DCX_BFORM(DCX_STORE_F32, DCX_COEF_A16)
+  DCX_BFORM(DCX_STORE_F32, DCX_COEF_N16)
   DCX_BFORM(DCX_STORE_F16, DCX_COEF_F16)
   DCX_BFORM(DCX_STORE_F16, DCX_COEF_U16)
 #undef DCX_BFORM
@@ -797,8 +856,10 @@ void launch_fast(const StepArgs &a, int tile_x, int tile_y, hipStream_t s) {
                 nz = a.box.z1 - a.box.z0;
   if (nx <= 0 || ny <= 0 || nz <= 0) return;
   if (a.zchunk <= 0) DCX_FAIL("z chunk must be positive");
-  // The kernel addresses every array with 32-bit byte offsets.
-  if ((a.box.z1 + 4) * a.zs * 4 > (int64_t(1) << 32) - 1)
+  // The kernel addresses every array with 32-bit byte offsets (the axis
+  // form's qword records take 8 bytes per cell).
+  const int64_t ebytes = a.cstore == DCX_COEF_N16 ? 8 : 4;
+  if ((a.box.z1 + 4) * a.zs * ebytes > (int64_t(1) << 32) - 1)
     DCX_FAIL("fast kernel needs arrays below 4 GiB (%lld planes of %lld cells)",
              (long long)(a.box.z1 + 4), (long long)a.zs);
   const int64_t gz = (nz + a.zchunk - 1) / a.zchunk;

@@ -125,6 +125,228 @@ __global__ void k_encode_coef(const float *a, uint16_t *out, int64_t n,
   }
 }
 
+// DCX_COEF_N16: the TTI axis n = (a, b, c) and the three v^2 ratios of one
+// cell, in double, from its ten FP32 coefficients.  n is found from the
+// largest diagonal term (its square root) and the two cross terms beside it;
+// n and -n give the same terms, so c >= 0.  v[3..5] = v2px/v2pz,
+// v2pn/v2pz, v2sz/v2pz where v2pz > 0 (ratio_ok), else 0.
+struct AxisCell {
+  double v[6];
+  bool ratio_ok;
+};
+
+__device__ AxisCell axis_of(const float c10[NCOEF]) {
+  const double xx = c10[CXX], yy = c10[CYY], zz = c10[CZZ];
+  const double xy = c10[CXY], yz = c10[CYZ], xz = c10[CXZ];
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (zz >= xx && zz >= yy) {
+    c = sqrt(zz);
+    if (c > 0.0) {
+      a = xz / (2.0 * c);
+      b = yz / (2.0 * c);
+    }
+  } else if (xx >= yy) {
+    a = sqrt(xx);
+    b = xy / (2.0 * a);
+    c = xz / (2.0 * a);
+  } else {
+    b = sqrt(yy);
+    a = xy / (2.0 * b);
+    c = yz / (2.0 * b);
+  }
+  if (c < 0.0) {
+    a = -a;
+    b = -b;
+    c = -c;
+  }
+  AxisCell r;
+  r.v[0] = a;
+  r.v[1] = b;
+  r.v[2] = c;
+  const double vz = c10[V2PZ];
+  r.ratio_ok = vz > 0.0;
+  r.v[3] = r.ratio_ok ? c10[V2PX] / vz : 0.0;
+  r.v[4] = r.ratio_ok ? c10[V2PN] / vz : 0.0;
+  r.v[5] = r.ratio_ok ? c10[V2SZ] / vz : 0.0;
+  return r;
+}
+
+struct Coef10 {
+  const float *c[NCOEF];  // pitched, element x = 0 of row 0
+};
+
+__device__ void load10(const Coef10 &cs, int64_t i, float c10[NCOEF]) {
+#pragma unroll
+  for (int k = 0; k < NCOEF; ++k) c10[k] = cs.c[k][i];
+}
+
+// Per block: min then max of the six axis-form values over the logical cells
+// (ratios over cells with v2pz > 0 only): out[12 * block + 2k + {0, 1}].
+__global__ void __launch_bounds__(256)
+    k_axis_range(Coef10 cs, int64_t sx, int64_t ys, int64_t nrows,
+                 double *out) {
+  double lo[6], hi[6];
+  for (int k = 0; k < 6; ++k) {
+    lo[k] = INFINITY;
+    hi[k] = -INFINITY;
+  }
+  const int64_t n = sx * nrows;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / sx;
+    float c10[NCOEF];
+    load10(cs, (i - row * sx) + row * ys, c10);
+    const AxisCell r = axis_of(c10);
+    for (int k = 0; k < 6; ++k) {
+      if (k >= 3 && !r.ratio_ok) continue;
+      lo[k] = fmin(lo[k], r.v[k]);
+      hi[k] = fmax(hi[k], r.v[k]);
+    }
+  }
+  __shared__ double s[12][256];
+  for (int k = 0; k < 6; ++k) {
+    s[2 * k][threadIdx.x] = lo[k];
+    s[2 * k + 1][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 6; ++k) {
+        s[2 * k][threadIdx.x] = fmin(s[2 * k][threadIdx.x], s[2 * k][threadIdx.x + w]);
+        s[2 * k + 1][threadIdx.x] =
+            fmax(s[2 * k + 1][threadIdx.x], s[2 * k + 1][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 12) out[12 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+struct AxisCode {
+  float lo[6], st[6];
+};
+
+__device__ __forceinline__ uint32_t axis_code(double v, float lo, float st) {
+  double t = rint((v - (double)lo) / (double)st);
+  t = t < 0.0 ? 0.0 : (t > 65535.0 ? 65535.0 : t);
+  return (uint32_t)t;
+}
+
+// Codes of every logical cell, and per block the largest error of the terms
+// the fast kernel rebuilds from them (its own float operations): absolute
+// for the six angle terms (out[2 block]), relative for the three v^2 terms
+// that are rebuilt (out[2 block + 1]).
+__global__ void __launch_bounds__(256)
+    k_axis_encode(Coef10 cs, int64_t sx, int64_t ys, int64_t nrows,
+                  AxisCode q, uint32_t *ab, uint64_t *cr, float *out) {
+  float e_ang = 0.0f, e_vel = 0.0f;
+  const int64_t n = sx * nrows;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / sx;
+    const int64_t j = (i - row * sx) + row * ys;
+    float c10[NCOEF];
+    load10(cs, j, c10);
+    const AxisCell r = axis_of(c10);
+    uint32_t u[6];
+    for (int k = 0; k < 6; ++k) u[k] = axis_code(r.v[k], q.lo[k], q.st[k]);
+    ab[j] = u[0] | (u[1] << 16);
+    cr[j] = (uint64_t)(u[2] | (u[3] << 16)) |
+            ((uint64_t)(u[4] | (u[5] << 16)) << 32);
+    float d[6];
+    for (int k = 0; k < 6; ++k) d[k] = fmaf((float)u[k], q.st[k], q.lo[k]);
+    const float c2 = d[2] + d[2];
+    const float ang[6] = {d[0] * d[0], d[1] * d[1], d[2] * d[2],
+                          (d[0] + d[0]) * d[1], c2 * d[1], c2 * d[0]};
+    for (int k = 0; k < 6; ++k) e_ang = fmaxf(e_ang, fabsf(ang[k] - c10[k]));
+    const float vz = c10[V2PZ];
+    const float vel[3] = {vz * d[3], vz * d[4], vz * d[5]};
+    const float ref[3] = {c10[V2PX], c10[V2PN], c10[V2SZ]};
+    for (int k = 0; k < 3; ++k) {
+      const float m = fabsf(ref[k]);
+      const float e = fabsf(vel[k] - ref[k]);
+      e_vel = fmaxf(e_vel, m > 0.0f ? e / m : (e > 0.0f ? INFINITY : 0.0f));
+    }
+  }
+  __shared__ float s[2][256];
+  s[0][threadIdx.x] = e_ang;
+  s[1][threadIdx.x] = e_vel;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s[0][threadIdx.x] = fmaxf(s[0][threadIdx.x], s[0][threadIdx.x + w]);
+      s[1][threadIdx.x] = fmaxf(s[1][threadIdx.x], s[1][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) out[2 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Build the DCX_COEF_N16 codes of st's FP32 coefficient arrays into st->C16
+// (allocated here).  Returns false, with st->C16 freed, when the arrays are
+// not of the TTI form within the stated error.
+bool encode_axis_form(State *st) {
+  const int blocks = kConvBlocks;
+  const dc_precomp_vars &pv = st->abi.precomp_vars;
+  Coef10 cs{{pv.ch1dxx, pv.ch1dyy, pv.ch1dzz, pv.ch1dxy, pv.ch1dyz, pv.ch1dxz,
+             pv.v2px, pv.v2pz, pv.v2sz, pv.v2pn}};
+  const int64_t sx = st->sizes[0], nrows = st->sizes[1] * st->sizes[2];
+  double *d_rng = nullptr;
+  DCX_CHECK(hipMalloc(&d_rng, 12 * blocks * sizeof(double)));
+  hipLaunchKernelGGL(k_axis_range, dim3(blocks), dim3(256), 0, st->stream, cs,
+                     sx, st->ys, nrows, d_rng);
+  DCX_CHECK(hipGetLastError());
+  std::vector<double> rng(12 * blocks);
+  DCX_CHECK(hipMemcpyAsync(rng.data(), d_rng, rng.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  DCX_CHECK(hipFree(d_rng));
+  AxisCode q;
+  for (int k = 0; k < 6; ++k) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < blocks; ++b) {
+      lo = std::min(lo, rng[12 * b + 2 * k]);
+      hi = std::max(hi, rng[12 * b + 2 * k + 1]);
+    }
+    if (!(lo <= hi)) lo = hi = 0.0;  // no cell with v2pz > 0
+    q.lo[k] = (float)lo;
+    q.st[k] = hi > lo ? (float)((hi - lo) / 65535.0) : 1.0f;
+    st->c_lo[k] = q.lo[k];
+    st->c_st[k] = q.st[k];
+  }
+  const size_t bytes = (size_t)st->alloc_floats * 12;
+  uint16_t *raw = nullptr;
+  DCX_CHECK(hipMalloc(&raw, bytes));
+  DCX_CHECK(hipMemsetAsync(raw, 0, bytes, st->stream));
+  st->C16 = raw + kPad;
+  float *d_err = nullptr;
+  DCX_CHECK(hipMalloc(&d_err, 2 * blocks * sizeof(float)));
+  hipLaunchKernelGGL(k_axis_encode, dim3(blocks), dim3(256), 0, st->stream, cs,
+                     sx, st->ys, nrows, q, n16_ab(st), n16_cr(st), d_err);
+  DCX_CHECK(hipGetLastError());
+  std::vector<float> err(2 * blocks);
+  DCX_CHECK(hipMemcpyAsync(err.data(), d_err, err.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  DCX_CHECK(hipFree(d_err));
+  float e_ang = 0.0f, e_vel = 0.0f;
+  for (int b = 0; b < blocks; ++b) {
+    e_ang = std::max(e_ang, err[2 * b]);
+    e_vel = std::max(e_vel, err[2 * b + 1]);
+  }
+  // NaN compares false: a NaN error fails too.
+  if (!(e_ang <= 1e-4f) || !(e_vel <= 1e-4f)) {
+    std::fprintf(stderr,
+                 "[dc_hip] DCX_COEF_N16: coefficients are not of the TTI axis "
+                 "form (largest angle-term error %.3g, v^2 relative error "
+                 "%.3g; bound 1e-4)\n",
+                 (double)e_ang, (double)e_vel);
+    DCX_CHECK(hipFree(raw));
+    st->C16 = nullptr;
+    return false;
+  }
+  return true;
+}
+
 uint16_t *alloc16(const State *st) {
   uint16_t *p = nullptr;
   const size_t bytes = (size_t)st->alloc_floats * 2;
@@ -209,10 +431,13 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
                      field_store == DCX_STORE_F16 ||
                      field_store == DCX_STORE_BF16;
   const bool cs_ok = coef_store == DCX_COEF_F32 || coef_store == DCX_COEF_F16 ||
-                     coef_store == DCX_COEF_U16 || coef_store == DCX_COEF_A16;
+                     coef_store == DCX_COEF_U16 || coef_store == DCX_COEF_A16 ||
+                     coef_store == DCX_COEF_N16;
   if (!fs_ok || !cs_ok) return -1;
   if (field_store == DCX_STORE_BF16 && coef_store == DCX_COEF_F32) return -1;
-  if (coef_store == DCX_COEF_A16 && field_store != DCX_STORE_F32) return -1;
+  if ((coef_store == DCX_COEF_A16 || coef_store == DCX_COEF_N16) &&
+      field_store != DCX_STORE_F32)
+    return -1;
   // A batch's launch tables and kernel follow the storage form the shots had
   // when the batch was made; converting one shot would mix element sizes.
   if (st->batch)
@@ -257,7 +482,13 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
   if (st->C16) DCX_CHECK(hipFree(st->C16 - kPad));
   st->C16 = nullptr;
   st->cstore = coef_store;
-  if (coef_store != DCX_COEF_F32) {
+  if (coef_store == DCX_COEF_N16) {
+    if (!encode_axis_form(st)) {
+      st->cstore = DCX_COEF_F32;
+      build_halo_tables(st);
+      return -1;
+    }
+  } else if (coef_store != DCX_COEF_F32) {
     DCX_CHECK(hipMalloc(&st->C16, (size_t)st->alloc_floats * 2 * NCOEF));
     st->C16 += kPad;
     const dc_precomp_vars &pv = st->abi.precomp_vars;
@@ -337,6 +568,10 @@ extern "C" int dcx_bytes_per_cell(const dc_device_data *data) {
   if (st->cstore == DCX_COEF_A16) {
     // Angle terms 2 B each, v^2 terms 4 B each, FP32 leapfrog fields.
     return 6 * fb + V2PX * 2 + (NCOEF - V2PX) * 4;
+  }
+  if (st->cstore == DCX_COEF_N16) {
+    // FP32 leapfrog fields; v2pz 4 B, six 16-bit codes 12 B.
+    return 6 * fb + 4 + 12;
   }
   // Leapfrog: read pc, qc, pp, qp, write pp, qp.  Time difference: read P and
   // D, write P and D, per field.

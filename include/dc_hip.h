@@ -80,9 +80,22 @@ void dcx_upload(dc_device_data *data, int which, const float *host);
  *                  angle terms, affine in log2 for the four v^2 terms (code
  *                  0 = exactly zero); ~14x finer than half on v^2.
  *  DCX_COEF_A16    the six angle terms as DCX_COEF_U16 codes them, the four
- *                  v^2 terms in FP32 (FP32 field storage only). */
+ *                  v^2 terms in FP32 (FP32 field storage only).
+ *  DCX_COEF_N16    the TTI form of the ten arrays (src/precomp.c:85-105):
+ *                  the six angle terms are n_i n_j (x2 off the diagonal) of
+ *                  the symmetry axis n = (sin t cos f, sin t sin f, cos t),
+ *                  and v2px, v2pn, v2sz are v2pz times 1 + 2 eps, 1 + 2 delta
+ *                  and (vsv/vpz)^2.  Stored as n's three components and the
+ *                  three ratios in 16-bit affine codes beside the FP32 v2pz
+ *                  array (16 B per cell instead of 40); the kernel rebuilds
+ *                  the ten terms in registers.  FP32 field storage only, and
+ *                  only for arrays of that form: dcx_set_storage returns -1
+ *                  when a decoded term would differ from the array's value by
+ *                  more than 1e-4 (angle terms, absolute) or 1e-4 relative
+ *                  (v^2 terms). */
 enum { DCX_STORE_F32 = 0, DCX_STORE_F16 = 1, DCX_STORE_BF16 = 2 };
-enum { DCX_COEF_F32 = 0, DCX_COEF_F16 = 1, DCX_COEF_U16 = 2, DCX_COEF_A16 = 3 };
+enum { DCX_COEF_F32 = 0, DCX_COEF_F16 = 1, DCX_COEF_U16 = 2, DCX_COEF_A16 = 3,
+       DCX_COEF_N16 = 4 };
 
 /* Convert the struct to the given storage (encoding the current fields and
  * coefficients; field_scale_log2 sets the field scale 2^k for half storage).

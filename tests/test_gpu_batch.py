@@ -58,7 +58,7 @@ def _alone(H, w, src, amp, steps, form):
     return f
 
 
-@pytest.mark.parametrize("form", [(0, 0), (0, 2), (0, 3), (1, 1)])
+@pytest.mark.parametrize("form", [(0, 0), (0, 2), (0, 3), (0, 4), (1, 1)])
 @pytest.mark.parametrize("steps,graph,tiling", [
     (24, False, (128, 1, 1)), (23, True, (16, 2, 2)), (7, False, (5, 1, 3)),
     (25, True, "auto"), (12, False, (64, 2, 0))])

@@ -38,6 +38,7 @@ def run40(H, O):
 
 @pytest.mark.parametrize("form,bpc,tol", [
     ((0, 0), 64, 1e-5), ((0, 1), 44, 1e-3), ((0, 2), 44, 1e-4), ((0, 3), 52, 1e-4),
+    ((0, 4), 40, 1e-4),
     ((1, 1), 36, 2e-2), ((1, 2), 36, 2e-2), ((2, 2), 36, 1e-1)])
 def test_storage_form_accuracy(H, run40, form, bpc, tol):
     coef, dt, steps, src, d, amp, f64 = run40
@@ -70,7 +71,7 @@ def test_storage_round_trip(H, run40):
     dd.close()
 
 
-@pytest.mark.parametrize("form", [(0, 2), (0, 3), (1, 1), (1, 2), (2, 2)])
+@pytest.mark.parametrize("form", [(0, 2), (0, 3), (0, 4), (1, 1), (1, 2), (2, 2)])
 def test_storage_forms_tile_shapes_bitwise_equal(H, run40, form):
     """A compact form computes each cell with the same expressions whatever
     the tile: 64x8 tiles (two apron cells per thread) give the bits of 32x16
@@ -88,3 +89,57 @@ def test_storage_forms_tile_shapes_bitwise_equal(H, run40, form):
     for a, b in zip(*out):
         assert np.array_equal(np.ascontiguousarray(a).view(np.uint32),
                               np.ascontiguousarray(b).view(np.uint32))
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def test_axis_form_refuses_arrays_of_another_form(H, run40):
+    """DCX_COEF_N16 assumes the reference's TTI structure: angle terms
+    n_i n_j of a unit axis, v^2 terms proportional to v2pz.  Arrays without
+    it (the ch1dxy term replaced by noise) are refused: set_storage returns
+    -1, the struct keeps its FP32 coefficients and runs as before, bit for
+    bit."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    bad = [c.copy() for c in coef]
+    rng = np.random.default_rng(3)
+    bad[3] = rng.uniform(-0.5, 0.5, bad[3].shape).astype(np.float32)
+    out = []
+    for try_axis in (True, False):
+        dd = H.DeviceData(bad, None, d, d, d, dt)
+        if try_axis:
+            assert not dd.set_storage(0, 4)
+            assert dd.bytes_per_cell() == 64
+        dd.run(steps, src, amp)
+        out.append(dd.get_results())
+        dd.close()
+    for a, b in zip(*out):
+        assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_constant_coefficient_arrays_code_exactly(H, O):
+    """The reference's default model (src/precomp.c:156-169) has the same
+    six angle terms in every cell.  Their 16-bit code ranges come from the
+    array's own cells (not its row padding), so each decodes to its FP32
+    value exactly, and 30 steps with the angle terms coded (DCX_COEF_A16)
+    give the bits of the FP32 arrays."""
+    n = 24
+    shape = (n, n, n)
+    vpz, vsv, eps, dlt, phi, th = [a.reshape(shape) for a in O.default_model(n ** 3)]
+    coef = O.precomp(vpz, vsv, eps, dlt, phi, th)
+    for k in range(6):
+        assert np.all(coef[k] == coef[k].flat[0])
+    dt = 0.002
+    src = 12 + n * (11 + n * 10)
+    amp = H.ricker_table(dt, 30)
+    out = []
+    for form in (0, 3):
+        dd = H.DeviceData(coef, None, 20.0, 20.0, 20.0, dt)
+        assert dd.set_storage(0, form)
+        dd.run(30, src, amp)
+        out.append(dd.get_results())
+        dd.close()
+    assert np.abs(out[0][0]).max() > 0
+    for a, b in zip(*out):
+        assert np.array_equal(_bits(a), _bits(b))
