@@ -486,28 +486,38 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   Staged cur = load_staged(zb - 4);
   Staged nx1 = cur;
   StepLoads lcur{}, lnx{};
-  if (PF >= 2) {
-    nx1 = load_staged(zb - 3);
+  // PF == 2 keeps the step values of plane n in stp[u & 1], a slot indexed
+  // by the plane's place u in the unrolled block of 8 (a constant in every
+  // copy), so that each load writes the registers its consumer reads.
+  StepLoads stp[2];
+  if (PF >= 2) nx1 = load_staged(zb - 3);
+  if (PF == 2) stp[0] = load_step(zb - 4);
+  if (PF >= 3) {
     lcur = load_step(zb - 4);
+    lnx = load_step(zb - 3);
   }
-  if (PF >= 3) lnx = load_step(zb - 3);
 
+  // Planes zb-4 .. zb+3 (the first block of 8) fill the z rings without
+  // output; every later plane n updates output plane j = n - 4.  The output
+  // decision is one per block, not one per plane.
   for (int nb = zb - 4; nb < nend; nb += 8) {
+    const bool out = nb != zb - 4;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int n = nb + u;
       if (n >= nend) break;
       // Stage plane n.
-      const float cp = F::dec(cur.cp), cq = F::dec(cur.cq);
+      const Staged sc = cur;
+      const float cp = F::dec(sc.cp), cq = F::dec(sc.cq);
       sP[lo] = cp;
       sQ[lo] = cq;
       if (la >= 0) {
-        sP[la] = F::dec(cur.ap);
-        sQ[la] = F::dec(cur.aq);
+        sP[la] = F::dec(sc.ap);
+        sQ[la] = F::dec(sc.aq);
       }
       if (kApron2 && la2 >= 0) {
-        sP[la2] = F::dec(cur.ap2);
-        sQ[la2] = F::dec(cur.aq2);
+        sP[la2] = F::dec(sc.ap2);
+        sQ[la2] = F::dec(sc.aq2);
       }
       __syncthreads();
 
@@ -520,10 +530,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         lcur = lnx;
         lnx = load_step(n + 2);
       } else if (PF == 2) {
-        L = lcur;
+        L = stp[u & 1];
         nxt = nx1;
         nx1 = load_staged(n + 2);
-        lcur = load_step(n + 1);
+        stp[(u + 1) & 1] = load_step(n + 1);
       } else {
         L = load_step(n);
         nxt = load_staged(n + 1);
@@ -550,7 +560,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       const float qxy = d1_lds(sDQ, ld, TX) * a.dxy;
 
       // Output plane j = n - 4 once the rings hold planes j-4 .. j+4.
-      if (n >= zb + 4) {
+      if (out) {
         const int J = (u + 4) & 7;  // slot of plane j
         const int O = u;            // slot of plane j-4 (about to be replaced)
 #define RS(r, b) r[(J + (b)) & 7]
