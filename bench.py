@@ -377,6 +377,10 @@ def pmc_child(args):
     VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
     w = make_workload(VM, args)
     d = open_device(H, w, args)
+    if args.child_form:
+        if not d.set_storage(args.child_form[0], args.child_form[1],
+                             FIELD_SCALE_LOG2):
+            raise SystemExit(f"storage form {args.child_form} refused")
     nbytes = d.copy_field(0, 1)  # known size: nbytes read, nbytes written
     d.synchronize()
     with open(os.environ["DC_BENCH_COPY_BYTES"], "w") as fh:
@@ -501,6 +505,10 @@ def main():
                     help="skip the rocprofv3 --pmc runs that measure HBM bytes "
                          "per stencil launch (traffic is then null)")
     ap.add_argument("--child-pmc", action="store_true", help=argparse.SUPPRESS)
+    # Counter child only: storage form (field, coefficient) of the stepped
+    # rank, for counter runs of the compact kernels.
+    ap.add_argument("--child-form", type=int, nargs=2, default=None,
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1:
