@@ -227,6 +227,40 @@ def time_forms(H, w, d, args, amp, cells):
     return out
 
 
+def precision_vs_fp32(H, VM, w, steps=1000):
+    """The compact forms' traces against the plain FP32 kernel's, GPU only:
+    the 136^3 crop of the benchmark's own model around its source, `steps`
+    steps, p at the 128x128 compute cells of the first plane below the
+    absorbing shell.  Relative L2 of each form's traces (and final p) from
+    the FP32 run's.  With the FP32 kernel's own trace error against FP64
+    (tests/test_gpu_precision.py asserts it), this bounds each form's error
+    against FP64 from above."""
+    c = VM.crop(w, (208, 208, 0), 136, receiver_z=36)
+    amp = H.ricker_table(c.dt, steps, c.fcut)
+    out = {"workload": f"136^3 crop of {w.name}, {steps} steps, 128x128 "
+                       f"receivers", "vs": "fp32 kernel"}
+    ref = None
+    for name, fs, cs in (("fp32", 0, 0), ("fp32_n16coef", 0, 4),
+                         ("fp32_a16coef", 0, 3), ("fp16", 1, 1)):
+        d = H.DeviceData(c.coef, None, c.dx, c.dy, c.dz, c.dt)
+        d.set_storage(fs, cs, FIELD_SCALE_LOG2)
+        tr, _ = d.run_traces(steps, c.src_index, amp, c.receivers)
+        pc, _ = d.get_results()
+        d.close()
+        if ref is None:
+            ref = (tr.astype(np.float64), pc.astype(np.float64))
+            continue
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        out[name] = {"trace_rel_l2": float(f"{rel(tr, ref[0]):.4g}"),
+                     "wavefield_rel_l2": float(f"{rel(pc, ref[1]):.4g}"),
+                     "finite": bool(np.isfinite(tr).all() and np.isfinite(pc).all())}
+        log(f"[bench] precision vs FP32 kernel, {steps} steps, {name}: "
+            f"traces {out[name]['trace_rel_l2']:.3e}")
+    return out
+
+
 def precision_study(H, VM, steps_list, log_prefix="[bench]"):
     """Relative L2 against the FP64 oracle (CPU) of the storage forms on a
     136^3 crop of the octant model around the source: every step's p at the
@@ -589,8 +623,11 @@ def main():
     if args.learned:
         learned = learned_study(H, args)
     precision = None
+    if not args.no_forms:
+        precision = {"vs_fp32_kernel": precision_vs_fp32(H, VM, w)}
     if args.precision_study:
-        precision = precision_study(H, VM, args.precision_study)
+        precision = precision or {}
+        precision["vs_fp64_oracle"] = precision_study(H, VM, args.precision_study)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, fields, args.cpu_steps)
@@ -636,6 +673,25 @@ def main():
         "learned_layer": learned,
         "precision": precision,
     }
+    # The metric's comparison: the fastest compact form whose traces stay
+    # within 1e-3 of the FP32 kernel's (FP32 is ~1e-5 from FP64), on the
+    # block and batched, against plain FP32.
+    if forms and precision and precision.get("vs_fp32_kernel"):
+        pv = precision["vs_fp32_kernel"]
+        ok = [k for k in forms if k != "fp32" and k in pv and pv[k]["finite"]
+              and pv[k]["trace_rel_l2"] <= 1e-3]
+        if ok:
+            best = max(ok, key=lambda k: forms[k]["mcell_per_s"])
+            mixed = {"form": best, "mcell_per_s": forms[best]["mcell_per_s"],
+                     "speedup_vs_fp32": forms[best]["speedup_vs_fp32"],
+                     "launch_speedup_vs_fp32": round(
+                         forms["fp32"]["kernel_ms"] / forms[best]["kernel_ms"], 3),
+                     "trace_rel_l2_vs_fp32_kernel": pv[best]["trace_rel_l2"]}
+            if batch and best in batch:
+                mixed["batched_mcell_per_s"] = batch[best]["batched"]
+                mixed["batched_speedup_vs_fp32_block"] = round(
+                    batch[best]["batched"] / forms["fp32"]["mcell_per_s"], 3)
+            line["mixed_vs_fp32"] = mixed
     print(json.dumps(line), flush=True)
 
 

@@ -327,6 +327,7 @@ double dcx_batch_run(dcx_batch *bb, unsigned int n_steps, const float *d4,
       else
         st->pcur ^= 1;
     }
+  for (State *st : b.shots) report_overflow(st);
   return (double)(ms + ms_total);
 }
 

@@ -78,6 +78,9 @@ struct StepArgs {
   int64_t czs;
   float c_lo[NCOEF], c_st[NCOEF];  // coefficient decode parameters
   float dscale;                    // time-difference units per field unit
+  // 16-bit field storage: device flag set when a stored value saturated or
+  // was not finite (null for FP32 fields).
+  int *ovf;
   int fstore, cstore;              // DCX_STORE_*, DCX_COEF_*
   int zchunk;     // planes per workgroup (fast kernel)
   int xcd_remap;  // give each XCD a contiguous run of tiles (fast kernel)
@@ -193,6 +196,10 @@ struct State {
   // field, 16-bit coefficients, and their decode parameters.
   int fstore, cstore;
   float fscale, dscale;
+  // Overflow flag of 16-bit field storage (device int, set by the kernels)
+  // and whether the host has reported it since the conversion.
+  int *ovf;
+  bool ovf_reported;
   uint16_t *P16[2][2];  // [field][level]
   uint16_t *D16[2];
   int pcur;             // level of P16 holding the current P
@@ -292,6 +299,11 @@ inline uint64_t *n16_cr(const State *st) {
          kPad;
 }
 void storage_free(State *st);
+// 16-bit field storage: whether a stored value saturated or was not finite
+// since the last conversion (waits for the struct's stream); report_overflow
+// prints a one-line warning the first time it is true.
+bool storage_overflowed(State *st);
+void report_overflow(State *st);
 void storage_download(const State *st, int which, float *host);
 void storage_add_source(State *st, int64_t dev_index, float amp);
 // (Re)build the device tables of the registered loopback halo boxes for the

@@ -106,6 +106,7 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
     a.of[0] = st->D16[0];
     a.of[1] = st->D16[1];
     a.dscale = st->dscale;
+    a.ovf = st->ovf;
   }
   for (int k = 0; k < NCOEF; ++k) {
     const bool f32 = st->cstore == DCX_COEF_F32 ||
@@ -572,6 +573,7 @@ void dc_device_data_get_results(dc_process_t *process, dc_device_data *data) {
   if (process->pc == nullptr || process->qc == nullptr)
     DCX_FAIL("process->pc/qc must be host buffers of the local size");
   if (st->fstore != DCX_STORE_F32) {
+    report_overflow(st);
     storage_download(st, 1, process->pc);
     storage_download(st, 3, process->qc);
     return;
@@ -721,7 +723,9 @@ int dcx_set_xcd_remap(dc_device_data *data, int enabled) {
 }
 
 void dcx_synchronize(dc_device_data *data) {
-  DCX_CHECK(hipStreamSynchronize(state_of(data)->stream));
+  State *st = state_of(data);
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  report_overflow(st);
 }
 
 static float *field_index(State *st, int which) {
@@ -739,6 +743,7 @@ void dcx_download(dc_device_data *data, int which, float *host) {
   if (st->fstore != DCX_STORE_F32) {
     if (which < 0 || which > 3) DCX_FAIL("field index %d not in 0..3", which);
     DCX_CHECK(hipStreamSynchronize(st->stream));
+    report_overflow(st);
     storage_download(st, which, host);
     return;
   }
@@ -961,6 +966,7 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
     DCX_CHECK(hipEventElapsedTime(&ms, st->ev[0], st->ev[1]));
   }
   (void)data;
+  report_overflow(st);
   return (double)ms;
 }
 

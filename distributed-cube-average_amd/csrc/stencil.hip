@@ -198,6 +198,7 @@ template <> struct FieldIO<DCX_STORE_F32> {
   static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
     stb<float>(b, o, v);
   }
+  static __device__ __forceinline__ bool bad(float) { return false; }
 };
 template <> struct FieldIO<DCX_STORE_F16> {
   static constexpr uint32_t kBytes = 2;
@@ -210,8 +211,15 @@ template <> struct FieldIO<DCX_STORE_F16> {
   static __device__ __forceinline__ float ld(const void *b, uint32_t o) {
     return dec(ldr(b, o));
   }
+  // Saturating store: a value beyond half's range is written as +-65504
+  // (NaN as -65504), never as inf or NaN; `bad` flags it for the
+  // struct's overflow counter.
   static __device__ __forceinline__ void st(void *b, uint32_t o, float v) {
-    stb<uint16_t>(b, o, __builtin_bit_cast(uint16_t, (_Float16)v));
+    const float c = fminf(fmaxf(v, -65504.0f), 65504.0f);
+    stb<uint16_t>(b, o, __builtin_bit_cast(uint16_t, (_Float16)c));
+  }
+  static __device__ __forceinline__ bool bad(float v) {
+    return !(fabsf(v) <= 65504.0f);
   }
 };
 template <> struct FieldIO<DCX_STORE_BF16> {
@@ -229,6 +237,10 @@ template <> struct FieldIO<DCX_STORE_BF16> {
     uint32_t u = __float_as_uint(v);
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
     stb<uint16_t>(b, o, (uint16_t)(u >> 16));
+  }
+  // bfloat16 has FP32's range: only a non-finite value is flagged.
+  static __device__ __forceinline__ bool bad(float v) {
+    return !(fabsf(v) <= 3.38953139e38f);
   }
 };
 
@@ -438,6 +450,9 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // Register rings (slot = plane index - (zb-4), mod 8 / mod 4).
   float rp[8], rq[8], xp[8], yp[8], xq[8], yq[8];
   float Sp[4], Hp[4], Sq[4], Hq[4];
+  // Whether a 16-bit store of this thread saturated (or met a non-finite
+  // value); counted once per wave in a.ovf at the end of the tile.
+  bool ovf = false;
 #pragma unroll
   for (int k = 0; k < 8; ++k) rp[k] = rq[k] = xp[k] = yp[k] = xq[k] = yq[k] = 0.f;
 #pragma unroll
@@ -639,6 +654,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
             F::st(Gqn, bo, qn);
             F::st(Gpp, bo, dnp);
             F::st(Gqp, bo, dnq);
+            ovf = ovf || F::bad(pn) || F::bad(qn) || F::bad(dnp) || F::bad(dnq);
           }
         } else {
           // (rhs * dt) * dt, the reference's association
@@ -680,6 +696,11 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 
       cur = nxt;
     }
+  }
+  // Only a run that has already left the form's range gets here with ovf
+  // set; the flag stays set until the storage is converted again.
+  if constexpr (kTimeDiff) {
+    if (ovf && a.ovf != nullptr) atomicOr(a.ovf, 1);
   }
 }
 

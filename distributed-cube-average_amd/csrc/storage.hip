@@ -26,7 +26,37 @@ __device__ __forceinline__ uint16_t enc_dev(int bf, float v) {
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
     return (uint16_t)(u >> 16);
   }
-  return __builtin_bit_cast(uint16_t, (_Float16)v);
+  // Saturating, as the stencil's stores: never inf or NaN.
+  return __builtin_bit_cast(uint16_t,
+                            (_Float16)fminf(fmaxf(v, -65504.0f), 65504.0f));
+}
+
+// Per block: max |a| and max |a - b| over the sx x nrows logical cells of
+// two pitched arrays (element x = 0 of row 0), out[2 block + {0, 1}].
+__global__ void __launch_bounds__(256)
+    k_absmax(const float *a, const float *b, int64_t sx, int64_t ys,
+             int64_t nrows, float *out) {
+  float m0 = 0.0f, m1 = 0.0f;
+  const int64_t n = sx * nrows;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / sx;
+    const int64_t j = (i - row * sx) + row * ys;
+    m0 = fmaxf(m0, fabsf(a[j]));
+    m1 = fmaxf(m1, fabsf(a[j] - b[j]));
+  }
+  __shared__ float s[2][256];
+  s[0][threadIdx.x] = m0;
+  s[1][threadIdx.x] = m1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s[0][threadIdx.x] = fmaxf(s[0][threadIdx.x], s[0][threadIdx.x + w]);
+      s[1][threadIdx.x] = fmaxf(s[1][threadIdx.x], s[1][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) out[2 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
 __device__ __forceinline__ float dec_dev(int bf, uint16_t v) {
@@ -384,6 +414,29 @@ void storage_free(State *st) {
   }
   if (st->C16) DCX_CHECK(hipFree(st->C16 - kPad));
   st->C16 = nullptr;
+  if (st->ovf) DCX_CHECK(hipFree(st->ovf));
+  st->ovf = nullptr;
+}
+
+bool storage_overflowed(State *st) {
+  if (st->ovf == nullptr || st->fstore == DCX_STORE_F32) return false;
+  int flag = 0;
+  DCX_CHECK(hipMemcpyAsync(&flag, st->ovf, sizeof(int), hipMemcpyDeviceToHost,
+                           st->stream));
+  DCX_CHECK(hipStreamSynchronize(st->stream));
+  return flag != 0;
+}
+
+void report_overflow(State *st) {
+  if (st->ovf_reported || !storage_overflowed(st)) return;
+  st->ovf_reported = true;
+  std::fprintf(stderr,
+               "[dc_hip] warning: %s field storage left its range (a stored "
+               "value saturated%s); the fields are not valid. Use FP32 "
+               "fields for this run (dcx_storage_overflow)\n",
+               st->fstore == DCX_STORE_F16 ? "FP16" : "BF16",
+               st->fstore == DCX_STORE_F16 ? " at +-65504 field units"
+                                           : " or was not finite");
 }
 
 // Host copy of field `which` (0 pp, 1 pc, 2 qp, 3 qc) of a compact struct:
@@ -460,10 +513,46 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
   }
   if (field_store != DCX_STORE_F32) {
     st->fstore = field_store;
-    st->fscale = field_store == DCX_STORE_F16
-                     ? std::ldexp(1.0f, field_scale_log2)
-                     : 1.0f;
+    // Scales: powers of two from the fields themselves, so that the largest
+    // |p|, |q| and |p_n - p_{n-1}| land at <= 2^12 field units, 16x below
+    // half's largest finite value; the caller's scale (and 2^3 for D) where
+    // the fields are still zero.  bfloat16 has FP32's range: scale 1, D x 8.
+    float maxp = 0.0f, maxd = 0.0f;
+    if (field_store == DCX_STORE_F16) {
+      float *d_mx = nullptr;
+      DCX_CHECK(hipMalloc(&d_mx, 2 * kConvBlocks * sizeof(float)));
+      std::vector<float> mx(2 * kConvBlocks);
+      for (int f = 0; f < 2; ++f) {
+        hipLaunchKernelGGL(k_absmax, dim3(kConvBlocks), dim3(256), 0,
+                           st->stream, f == 0 ? st->abi.pc : st->abi.qc,
+                           f == 0 ? st->abi.pp : st->abi.qp, st->sizes[0],
+                           st->ys, st->sizes[1] * st->sizes[2], d_mx);
+        DCX_CHECK(hipGetLastError());
+        DCX_CHECK(hipMemcpyAsync(mx.data(), d_mx, mx.size() * sizeof(float),
+                                 hipMemcpyDeviceToHost, st->stream));
+        DCX_CHECK(hipStreamSynchronize(st->stream));
+        for (int b = 0; b < kConvBlocks; ++b) {
+          maxp = std::max(maxp, mx[2 * b]);
+          maxd = std::max(maxd, mx[2 * b + 1]);
+        }
+      }
+      DCX_CHECK(hipFree(d_mx));
+    }
+    constexpr float kHead = 4096.0f;
+    st->fscale = 1.0f;
     st->dscale = 8.0f;
+    if (field_store == DCX_STORE_F16) {
+      st->fscale = maxp > 0.0f
+                       ? std::ldexp(1.0f, (int)std::floor(std::log2(kHead / maxp)))
+                       : std::ldexp(1.0f, field_scale_log2);
+      if (maxd > 0.0f) {
+        const int j = (int)std::floor(std::log2(kHead / (maxd * st->fscale)));
+        st->dscale = std::ldexp(1.0f, std::min(16, std::max(0, j)));
+      }
+    }
+    if (st->ovf == nullptr) DCX_CHECK(hipMalloc(&st->ovf, sizeof(int)));
+    DCX_CHECK(hipMemsetAsync(st->ovf, 0, sizeof(int), st->stream));
+    st->ovf_reported = false;
     st->pcur = 0;
     for (int f = 0; f < 2; ++f) {
       st->P16[f][0] = alloc16(st);
@@ -559,6 +648,10 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
   DCX_CHECK(hipStreamSynchronize(st->stream));
   build_halo_tables(st);
   return 0;
+}
+
+extern "C" int dcx_storage_overflow(dc_device_data *data) {
+  return storage_overflowed(state_of(data)) ? 1 : 0;
 }
 
 extern "C" int dcx_bytes_per_cell(const dc_device_data *data) {

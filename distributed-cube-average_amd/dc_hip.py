@@ -98,6 +98,7 @@ SIGNATURES = {
     "dcx_synchronize": (None, [_DD]),
     "dcx_set_storage": (C.c_int, [_DD, C.c_int, C.c_int, C.c_int]),
     "dcx_bytes_per_cell": (C.c_int, [_DD]),
+    "dcx_storage_overflow": (C.c_int, [_DD]),
     "dcx_download": (None, [_DD, C.c_int, _FP]),
     "dcx_upload": (None, [_DD, C.c_int, _FP]),
     "dcx_step": (None, [_DD, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -369,6 +370,11 @@ class DeviceData:
         """Switch to a compact storage form (see include/dc_hip.h)."""
         return load().dcx_set_storage(self.ptr, field_store, coef_store,
                                       field_scale_log2) == 0
+
+    def storage_overflow(self) -> bool:
+        """dcx_storage_overflow: whether a 16-bit field store has saturated
+        since the last conversion."""
+        return bool(load().dcx_storage_overflow(self.ptr))
 
     def bytes_per_cell(self) -> int:
         return load().dcx_bytes_per_cell(self.ptr)

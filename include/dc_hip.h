@@ -105,6 +105,21 @@ enum { DCX_COEF_F32 = 0, DCX_COEF_F16 = 1, DCX_COEF_U16 = 2, DCX_COEF_A16 = 3,
 int dcx_set_storage(dc_device_data *data, int field_store, int coef_store,
                     int field_scale_log2);
 
+/* Half field storage is range-checked.  At conversion (dcx_set_storage) the
+ * field scale and the time-difference factor are powers of two chosen from
+ * the fields' own largest |p|, |q| and |p_n - p_{n-1}| so that they sit at
+ * <= 2^12 field units, 16x below half's largest finite value (the caller's
+ * field_scale_log2, and 2^3 for the difference, where the fields are still
+ * zero).  Every 16-bit store saturates at +-65504 instead of becoming inf or
+ * NaN, and sets a device flag when it did (bfloat16: when the value was not
+ * finite).  dcx_storage_overflow returns 1 when any store has done so since
+ * the last conversion (it waits for the struct's queued work), else 0;
+ * dcx_run, dcx_synchronize, dcx_download and dc_device_data_get_results print
+ * a one-line warning on stderr the first time it happens.  The half field
+ * forms' rounding makes the leapfrog grow slowly (about 2x per 700 steps after
+ * ~1500 steps on the octant model), so long runs do reach the range. */
+int dcx_storage_overflow(dc_device_data *data);
+
 /* Bytes per cell-update the fast kernel moves (algorithmic, no apron) for
  * the struct's current storage. */
 int dcx_bytes_per_cell(const dc_device_data *data);

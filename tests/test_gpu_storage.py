@@ -143,3 +143,43 @@ def test_constant_coefficient_arrays_code_exactly(H, O):
     assert np.abs(out[0][0]).max() > 0
     for a, b in zip(*out):
         assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_half_storage_saturates_and_reports(H, run40, capfd):
+    """A source far beyond half's range at the caller's scale: the stored
+    values saturate at +-65504 field units instead of becoming inf or NaN,
+    dcx_storage_overflow reports it, and the next synchronizing call prints
+    one warning."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    dd = H.DeviceData(coef, None, d, d, d, dt)
+    assert dd.set_storage(1, 1, 8)
+    assert not dd.storage_overflow()
+    dd.run(steps, src, (1e5 * amp).astype(np.float32))
+    pc, qc = dd.get_results()
+    flagged = dd.storage_overflow()
+    dd.close()
+    err = capfd.readouterr().err
+    assert flagged
+    assert np.isfinite(pc).all() and np.isfinite(qc).all()
+    assert np.abs(pc).max() <= 65504.0 / 256.0 * (1 + 1e-6)
+    assert err.count("field storage left its range") == 1
+
+
+def test_half_storage_scale_follows_the_fields(H, run40):
+    """Converting fields whose values are far beyond 65504 / 2^8 (the
+    caller's scale) picks the scale from the fields: no saturation, and the
+    converted fields equal the FP32 ones to half precision."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    dd = H.DeviceData(coef, None, d, d, d, dt)
+    dd.run(steps, src, (1e4 * amp).astype(np.float32))
+    before = dd.fields()
+    big = np.abs(before[1]).max()
+    assert big * 256.0 > 65504.0
+    assert dd.set_storage(1, 1, 8)
+    after = dd.fields()
+    flagged = dd.storage_overflow()
+    dd.close()
+    assert not flagged
+    for k in (1, 3):
+        assert np.isfinite(after[k]).all()
+        assert np.abs(after[k] - before[k]).max() <= 2e-3 * big
