@@ -150,6 +150,46 @@ def test_full_size_survey_shots(H, VM):
             assert np.array_equal(bits(g), bits(e)), (k, name)
 
 
+@pytest.mark.parametrize("form", [(0, 0), (0, 4)])
+def test_whole_survey_batched_equals_shots_alone(H, VM, form):
+    """The batching configuration at its workload: all 64 shots of the
+    benchmark survey (compute extents U[128,320] x U[128,320] x U[128,256],
+    589 M cells per step), each with its own Ricker, in one persistent
+    launch per step with the autotuned tiling and a graph, FP32 and the
+    axis coefficient form.  A few steps (the autotuner's trials included),
+    and every shot's four fields bit-identical to the shot stepped alone."""
+    S = VM.layered_survey(seed=0, n_shots=64, steps=40)
+    steps = 16
+    ws, ds = [], []
+    for k in range(len(S)):
+        w = S.shot(k)
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        assert d.set_storage(*form)
+        d.release_host()
+        w.coef = None  # the shot is rebuilt below for its run alone
+        ws.append(w)
+        ds.append(d)
+    say("survey", len(ds), "shots on the device, form", form)
+    idx = [w.src_index for w in ws]
+    amp = np.stack([H.ricker_table(w.dt, steps, w.fcut) for w in ws])
+    b = H.Batch(ds)
+    b.set_autotune(True)
+    b.run(steps, idx, amp, graph=True)
+    say("  tiling used", b.tiling())
+    b.close()
+    for k, d in enumerate(ds):
+        got = d.fields()
+        d.close()
+        w = S.shot(k)
+        a = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        assert a.set_storage(*form)
+        a.run(steps, idx[k], amp[k], graph=True)
+        want = a.fields()
+        a.close()
+        for name, g, e in zip(("pp", "pc", "qp", "qc"), got, want):
+            assert np.array_equal(bits(g), bits(e)), (k, name)
+
+
 def test_freeing_a_shot_inside_a_batch_fails_loudly(H):
     """A shot freed while its batch is open would leave the batch with a
     dangling struct: the library refuses (message and exit), and a network

@@ -286,3 +286,29 @@ def test_shipped_layer_gains_over_the_layer_without_it(H, pkg):
     assert r["learned"] < r["none"] / 4
     assert r["learned"] < 0.5 * r["sponge6"]
     mlp.close()
+
+
+def test_shipped_layer_at_the_384_workload(H, pkg):
+    """The learned-layer configuration at its workload: a 384^3 interior with
+    the shipped 6-cell layer, scored against the 640^3 padded reference run
+    (128-cell margin), source 40 cells below one face, 1000 steps, seed 7001;
+    beside it the same domain with nothing at its edge, with the reference's
+    6-cell random-velocity shell, and grown to the 32-cell shell.  Measured
+    when the weights were made: learned 0.028, 32-cell shell 0.065, 6-cell
+    shell 0.167, nothing 0.333 (the layer loses to the 32-cell shell on seeds
+    7002 and 7003, profiles/learned_layer_w6.md)."""
+    import importlib
+    LB = importlib.import_module("distributed-cube-average_amd.learned_boundary")
+    params, meta = LB.load_weights()
+    mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
+    say("learned layer 384^3 in a 640^3 padded run, seed 7001")
+    case = LB.make_case(7001, inner=384, width=int(meta["width"]), margin=128,
+                        src_depth=40)
+    r = LB.reflection(mlp, case, 1000)
+    mlp.close()
+    say("384^3 reflection ratios:", {k: f"{v:.3e}" for k, v in r.items()})
+    for v in r.values():
+        assert np.isfinite(v)
+    assert r["learned"] < r["none"] / 4
+    assert r["learned"] < 0.5 * r["sponge6"]
+    assert r["learned"] < r["sponge32"]
