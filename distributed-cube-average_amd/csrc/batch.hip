@@ -151,7 +151,18 @@ struct Plan {
     std::vector<StepArgs> args[2];
     std::vector<int64_t> prefix(n + 1, 0);
     const State *s0 = b.shots[0];
-    for (int k = 0; k < n; ++k) {
+    // Tiles are dealt to the compute units in table order, so the shots
+    // with the longest tiles (most planes per z chunk) go first: the last
+    // tiles of a step are then short ones and the step ends evenly.
+    std::vector<int> ord(n);
+    for (int k = 0; k < n; ++k) ord[k] = k;
+    auto planes = [&](int k) {
+      return std::min<int64_t>(zchunk, b.shots[k]->sizes[2] - 2 * STENCIL);
+    };
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int i, int j) { return planes(i) > planes(j); });
+    for (int j = 0; j < n; ++j) {
+      const int k = ord[j];
       State *st = b.shots[k];
       // The entry points that could change these refuse batch shots; the
       // launch picks one kernel for all shots from shot 0's form.
@@ -188,7 +199,7 @@ struct Plan {
       const int64_t ebytes = st->cstore == DCX_COEF_N16 ? 8 : 4;
       if ((int64_t)(box.z1 + 4) * st->zs * ebytes > (int64_t(1) << 32) - 1)
         DCX_FAIL("shot %d arrays exceed 4 GiB", k);
-      prefix[k + 1] = prefix[k] + fast_tiles(args[0][k], 32, 16);
+      prefix[j + 1] = prefix[j] + fast_tiles(args[0][j], 32, 16);
     }
     DCX_CHECK(hipMemcpy(d_args, args[0].data(), n * sizeof(StepArgs),
                         hipMemcpyHostToDevice));
