@@ -32,27 +32,6 @@ static void usage(const char *prog) {
   std::exit(64);
 }
 
-// Balanced 3-D factorisation of n, largest factor first (the shape
-// MPI_Dims_create returns for a free 3-D grid).
-static void dims_create(int n, int dims[3]) {
-  int best = -1;
-  for (int a = 1; a <= n; ++a) {
-    if (n % a) continue;
-    for (int b = 1; b <= a; ++b) {
-      if ((n / a) % b) continue;
-      const int c = n / a / b;
-      if (c > b) continue;
-      const int spread = a - c;
-      if (best < 0 || spread < best) {
-        best = spread;
-        dims[0] = a;
-        dims[1] = b;
-        dims[2] = c;
-      }
-    }
-  }
-}
-
 int main(int argc, char **argv) {
   dcx_program_args a{};
   a.kernel = DCX_KERNEL_FAST;
@@ -111,7 +90,7 @@ int main(int argc, char **argv) {
   if (topo[0] > 0) {
     for (int k = 0; k < 3; ++k) a.topology[k] = topo[k];
   } else {
-    dims_create(ranks, a.topology);
+    dcx_dims_create(ranks, a.topology);
   }
   const size_t gx = a.size_x + 2 * a.absorption + 8;
   const size_t gy = a.size_y + 2 * a.absorption + 8;

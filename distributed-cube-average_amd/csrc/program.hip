@@ -44,6 +44,34 @@ void axis_box(int d, int64_t s, bool recv, int64_t &a, int64_t &b) {
 
 using namespace dcx;
 
+// MPI_Dims_create(n, 3, dims) with dims zeroed, as src/main.c:89-90 calls it:
+// of the factorisations n = a*b*c with a >= b >= c, the one with the least
+// spread a - c, and of equal spreads the one with the larger c (MPICH's
+// balanced decomposition; n = 360 gives 10,6,6, not 9,8,5).
+extern "C" int dcx_dims_create(int n, int dims[3]) {
+  if (n < 1 || dims == nullptr) return -1;
+  int best_spread = -1, best_c = 0;
+  for (int a = 1; a <= n; ++a) {
+    if (n % a) continue;
+    const int m = n / a;
+    for (int b = 1; b <= a; ++b) {
+      if (m % b) continue;
+      const int c = m / b;
+      if (c > b) continue;
+      const int spread = a - c;
+      if (best_spread < 0 || spread < best_spread ||
+          (spread == best_spread && c > best_c)) {
+        best_spread = spread;
+        best_c = c;
+        dims[0] = a;
+        dims[1] = b;
+        dims[2] = c;
+      }
+    }
+  }
+  return 0;
+}
+
 extern "C" int dcx_program_run(const dcx_program_args *args, float *out_pc,
                                float *out_qc, double stats[3]) {
   if (args == nullptr) return -1;

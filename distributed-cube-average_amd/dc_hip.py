@@ -154,6 +154,7 @@ SIGNATURES = {
                                   C.POINTER(C.c_double)]),
     "dcx_write_dc_file": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t,
                                     C.c_size_t, _FP, _FP]),
+    "dcx_dims_create": (C.c_int, [C.c_int, _I3]),
     "dcx_default_model": (None, [C.c_size_t] + [_FP] * 6),
     "dcx_precompute": (None, [C.c_size_t] + [_FP] * 6 + [_FP * 10]),
     "dcx_device_data_init_model": (_DD, [C.POINTER(Process), _I3, _I3, _I3,
@@ -681,6 +682,15 @@ def learned_sample_step(padded: DeviceData, small: DeviceData, origin, width,
         padded.ptr, small.ptr, (C.c_longlong * 3)(*origin), width, n_cells,
         seed & 0xffffffff, samples.ptr, padded.dx, padded.dy, padded.dz,
         padded.dt, src_index, src_next, int(add_next), flags)
+
+
+def dims_create(n):
+    """The rank topology the reference program uses for n ranks
+    (src/main.c:89-90, MPI_Dims_create with zeroed dims); host only."""
+    d = _I3(0, 0, 0)
+    if load().dcx_dims_create(int(n), d) != 0:
+        raise ValueError(f"dims_create: n must be >= 1, got {n}")
+    return tuple(d)
 
 
 def program_run(size, absorption, d, dt, time_max, topology=(1, 1, 1),

@@ -404,6 +404,13 @@ typedef struct {
 int dcx_program_run(const dcx_program_args *args, float *out_pc,
                     float *out_qc, double stats[3]);
 
+/* The rank topology the reference program takes for n ranks: src/main.c:89-90
+ * calls MPI_Dims_create(n, 3, dims) with dims zeroed.  Of the factorisations
+ * n = dims[0]*dims[1]*dims[2] with dims non-increasing, the one with the least
+ * dims[0] - dims[2], and of equal spreads the larger dims[2] (MPICH's
+ * result).  Host only, no device work.  Returns 0, or -1 for n < 1. */
+int dcx_dims_create(int n, int dims[3]);
+
 /* Write the two global fields as the reference's output file: pc then qc,
  * float32, x fastest (src/coordinator.c:116-219).  Returns 0 on success. */
 int dcx_write_dc_file(const char *path, size_t gx, size_t gy, size_t gz,

@@ -21,9 +21,16 @@ then records inputs and outputs as small .npz fixtures:
   golden_dc_*.npz     the reference program bin/dc (src/main.c) under mpirun
                       with 1 and 8 ranks; the .dc output (pc then qc, global
                       x-fastest float32) plus the command line.
+  golden_dims.npz     MPI_Dims_create(n, 3, {0,0,0}) for n = 1..4096, the
+                      call src/main.c:89-90 makes, from the MPICH library
+                      the reference links (its version string recorded).
 
-The reference never runs anywhere else: the GPU-side tests read only these
-fixtures and the oracle.
+`python tests/golden/make_golden.py --dims` writes golden_dims.npz alone.
+
+Tests read these fixtures and the oracle.  Of the reference itself only one
+compiled part leaves this container: oracle/_ref/libdcref.so, its OpenMP
+dc_propagate built from its sources by oracle/Makefile, which travels with the
+tree and is the CPU baseline bench.py times beside the GPU.
 """
 from __future__ import annotations
 
@@ -40,6 +47,45 @@ sys.path.insert(0, os.path.join(REPO, "oracle"))
 import oracle as O  # noqa: E402
 
 MPIRUN = "/opt/conda/bin/mpirun"
+MPI_HOME = "/opt/conda"
+
+DIMS_PROGRAM = r"""
+#include <mpi.h>
+#include <stdio.h>
+int main(int argc, char **argv) {
+  char version[MPI_MAX_LIBRARY_VERSION_STRING];
+  int len = 0;
+  MPI_Init(&argc, &argv);
+  MPI_Get_library_version(version, &len);
+  printf("%s\n", version);
+  for (int n = 1; n <= 4096; ++n) {
+    int d[3] = {0, 0, 0};
+    MPI_Dims_create(n, 3, d);
+    printf("%d %d %d %d\n", n, d[0], d[1], d[2]);
+  }
+  MPI_Finalize();
+  return 0;
+}
+"""
+
+
+def write_dims():
+    """golden_dims.npz: the MPI library's own MPI_Dims_create outputs."""
+    with tempfile.TemporaryDirectory() as td:
+        src, exe = os.path.join(td, "dims.c"), os.path.join(td, "dims")
+        with open(src, "w") as f:
+            f.write(DIMS_PROGRAM)
+        lib = os.path.join(MPI_HOME, "lib")
+        subprocess.run(["gcc", "-O1", "-I", os.path.join(MPI_HOME, "include"),
+                        src, "-o", exe, "-L", lib, "-lmpi",
+                        "-Wl,-rpath," + lib], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True,
+                             timeout=120).stdout.splitlines()
+    rows = [line.split() for line in out if line[:1].isdigit()]
+    version = next(line for line in out if not line[:1].isdigit()).strip()
+    a = np.array(rows, np.int32)
+    np.savez_compressed(os.path.join(HERE, "golden_dims.npz"), n=a[:, 0],
+                        dims=a[:, 1:], mpi_version=np.array(version))
 
 
 def random_model(rng, shape):
@@ -187,10 +233,14 @@ def main():
                     absorption=np.array(c["a"]), d=np.float32(c["d"]),
                     dt=np.array(c["dt"]), tmax=np.array(c["tmax"]),
                     ranks=np.array(np_), argv=np.array(" ".join(cmd[4:])))
+    write_dims()
     for fn in sorted(os.listdir(HERE)):
         if fn.endswith(".npz"):
             print(fn, os.path.getsize(os.path.join(HERE, fn)))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--dims"]:
+        write_dims()
+    else:
+        main()

@@ -28,6 +28,20 @@ def test_missing_or_zero_flag_prints_usage(argv):
     assert "Usage" in r.stderr
 
 
+def test_rank_topology_is_mpi_dims_create(H):
+    """--ranks R takes the topology src/main.c:89-90 gets from
+    MPI_Dims_create(R, 3, {0,0,0}): every R up to 4096 against the MPI
+    library's own outputs (golden_dims.npz), including the equal-spread ties
+    (R = 360 is 10,6,6, not 9,8,5)."""
+    g = golden("golden_dims.npz")
+    got = np.array([H.dims_create(int(n)) for n in g["n"]])
+    bad = np.flatnonzero((got != g["dims"]).any(axis=1))
+    assert bad.size == 0, [(int(g["n"][i]), got[i], g["dims"][i]) for i in bad[:5]]
+    assert H.dims_create(360) == (10, 6, 6)
+    with pytest.raises(ValueError):
+        H.dims_create(0)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("ranks", [1, 8])
 def test_program_writes_reference_output_file(tmp_path, ranks):
