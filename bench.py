@@ -296,9 +296,11 @@ def precision_study(H, VM, steps_list, log_prefix="[bench]"):
             d.set_storage(fs, cs, FIELD_SCALE_LOG2)
             tr, _ = d.run_traces(steps, w.src_index, amp, w.receivers)
             pc, _ = d.get_results()
+            flagged = d.storage_overflow()
             d.close()
             row[name] = {"trace_rel_l2": rel(tr, tr64),
-                         "wavefield_rel_l2": rel(pc, f64[1])}
+                         "wavefield_rel_l2": rel(pc, f64[1]),
+                         "left_half_range": float(flagged)}
             log(f"{log_prefix} precision {steps} steps {name}: traces "
                 f"{row[name]['trace_rel_l2']:.3e} wavefield "
                 f"{row[name]['wavefield_rel_l2']:.3e}")
