@@ -118,10 +118,11 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   }
   if (st->cstore == DCX_COEF_N16) {
     // The axis form's dword (a, b) and qword (c, rx, rn, rs) code arrays
-    // share the C16 allocation; v2pz stays the FP32 array.
+    // share the C16 allocation; v2pz stays the FP32 array.  The kernel reads
+    // (a, b) once per cell, through slot CXX.
     const void *ab = n16_ab(st), *cr = n16_cr(st);
     for (int k = 0; k < NCOEF; ++k) a.cf[k] = nullptr;
-    a.cf[CXX] = a.cf[CZZ] = ab;
+    a.cf[CXX] = ab;
     a.cf[CXZ] = a.cf[CYZ] = cr;
     a.cf[V2PZ] = a.coef.c[V2PZ];
   }

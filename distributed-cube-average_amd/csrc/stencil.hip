@@ -303,17 +303,17 @@ template <> struct CoefIO<DCX_COEF_A16> {
 
 // TTI axis form: per cell a dword AB = (a, b) codes, a qword CR = (c, rx, rn,
 // rs) codes and the FP32 v2pz array.  The in-plane part of H1 needs a and b
-// of the staged plane (raw slot cxx = AB); the output plane needs AB (raw
-// slot czz), CR (slots cxz, cyz) and v2pz (slot vpz).  Offsets are formed
-// for 4-byte elements and doubled for CR (every array below 4 GiB, checked
-// at launch).  Decode parameters: c_lo/c_st[0..5] for a, b, c, rx, rn, rs.
+// of the staged plane (raw slot cxx = AB), which the kernel then keeps for
+// the plane's output 4 planes later; the output plane needs CR (slots cxz,
+// cyz) and v2pz (slot vpz).  Offsets are formed for 4-byte elements and
+// doubled for CR (every array below 4 GiB, checked at launch).  Decode
+// parameters: c_lo/c_st[0..5] for a, b, c, rx, rn, rs.
 template <> struct CoefIO<DCX_COEF_N16> {
   static constexpr uint32_t kBytes = 4;
   static __device__ __forceinline__ uint32_t ldr(const StepArgs &a, int k,
                                                  uint32_t o) {
     switch (k) {
       case CXX: return ldb<uint32_t>(a.cf[CXX], o);
-      case CZZ: return ldb<uint32_t>(a.cf[CZZ], o);
       case CXZ: return ldb<uint32_t>(a.cf[CXZ], 2u * o);
       case CYZ: return ldb<uint32_t>(a.cf[CXZ], 2u * o + 4u);
       case V2PZ: return ldb<uint32_t>(a.cf[V2PZ], o);
@@ -450,13 +450,15 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // Register rings (slot = plane index - (zb-4), mod 8 / mod 4).
   float rp[8], rq[8], xp[8], yp[8], xq[8], yq[8];
   float Sp[4], Hp[4], Sq[4], Hq[4];
+  // Axis form: 2a and b of the staged planes' cells, kept for their output.
+  float Na2[4], Nb[4];
   // Whether a 16-bit store of this thread saturated (or met a non-finite
   // value); counted once per wave in a.ovf at the end of the tile.
   bool ovf = false;
 #pragma unroll
   for (int k = 0; k < 8; ++k) rp[k] = rq[k] = xp[k] = yp[k] = xq[k] = yq[k] = 0.f;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) Sp[k] = Hp[k] = Sq[k] = Hq[k] = 0.f;
+  for (int k = 0; k < 4; ++k) Sp[k] = Hp[k] = Sq[k] = Hq[k] = Na2[k] = Nb[k] = 0.f;
 
   const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
   auto load_staged = [&](int k) {
@@ -604,37 +606,41 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
                            W_L3 * (RS(yq, 3) - RS(yq, 5)) +
                            W_L4 * (dyq - yq[O])) * a.dyz;
 #undef RS
-        float czz, cxz, cyz, vpx, vpz, vsz, vpn;
+        const int D = u & 3;
+        float rhsp, rhsq;
         if constexpr (CS == DCX_COEF_N16) {
-          // n = (na, nb, nc): czz = nc^2, cxz = 2 na nc, cyz = 2 nb nc;
-          // v2px, v2pn, v2sz = v2pz times the three ratios.
-          const float na = Cf::lo(a, 0, L.czz), nb = Cf::hi(a, 1, L.czz);
+          // n = (na, nb, nc): czz = nc^2, cxz = 2 na nc, cyz = 2 nb nc, with
+          // 2na and nb kept from the plane's staging.  v2px, v2pn, v2sz are
+          // v2pz times the ratios rx, rn, rs, so v2pz factors out:
+          //   rhsp = v2pz (rx H2p + H1q + rs (H1p - H1q)),
+          //   rhsq = v2pz (rn H2p + H1q - rs (H2p - H2q)).
           const float nc = Cf::lo(a, 2, L.cxz);
-          const float c2 = nc + nc;
-          czz = nc * nc;
-          cxz = c2 * na;
-          cyz = c2 * nb;
-          vpz = __uint_as_float(L.vpz);
-          vpx = vpz * Cf::hi(a, 3, L.cxz);
-          vpn = vpz * Cf::lo(a, 4, L.cyz);
-          vsz = vpz * Cf::hi(a, 5, L.cyz);
+          const float czz = nc * nc, cxz = nc * Na2[D], cyz = (nc + nc) * Nb[D];
+          const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
+          const float h2p = Sp[D] + pzz - h1p;
+          const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
+          const float h2q = Sq[D] + qzz - h1q;
+          const float vpz = __uint_as_float(L.vpz);
+          const float rx = Cf::hi(a, 3, L.cxz), rn = Cf::lo(a, 4, L.cyz);
+          const float rs = Cf::hi(a, 5, L.cyz);
+          rhsp = vpz * (rx * h2p + h1q + rs * (h1p - h1q));
+          rhsq = vpz * (rn * h2p + h1q - rs * (h2p - h2q));
         } else {
-          czz = Cf::dec(a, CZZ, L.czz);
-          cxz = Cf::dec(a, CXZ, L.cxz);
-          cyz = Cf::dec(a, CYZ, L.cyz);
-          vpx = Cf::dec(a, V2PX, L.vpx);
-          vpz = Cf::dec(a, V2PZ, L.vpz);
-          vsz = Cf::dec(a, V2SZ, L.vsz);
-          vpn = Cf::dec(a, V2PN, L.vpn);
+          const float czz = Cf::dec(a, CZZ, L.czz);
+          const float cxz = Cf::dec(a, CXZ, L.cxz);
+          const float cyz = Cf::dec(a, CYZ, L.cyz);
+          const float vpx = Cf::dec(a, V2PX, L.vpx);
+          const float vpz = Cf::dec(a, V2PZ, L.vpz);
+          const float vsz = Cf::dec(a, V2SZ, L.vsz);
+          const float vpn = Cf::dec(a, V2PN, L.vpn);
+          const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
+          const float h2p = Sp[D] + pzz - h1p;
+          const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
+          const float h2q = Sq[D] + qzz - h1q;
+          rhsp = vpx * h2p + vpz * h1q + vsz * (h1p - h1q);
+          rhsq = vpn * h2p + vpz * h1q - vsz * (h2p - h2q);
         }
         const float opp = F::dec(L.opp), oqp = F::dec(L.oqp);
-        const int D = u & 3;
-        const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
-        const float h2p = Sp[D] + pzz - h1p;
-        const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
-        const float h2q = Sq[D] + qzz - h1q;
-        const float rhsp = vpx * h2p + vpz * h1q + vsz * (h1p - h1q);
-        const float rhsq = vpn * h2p + vpz * h1q - vsz * (h2p - h2q);
         const int j = n - 4;
         const bool src = j == a.src_z && (int64_t)own == a.src_xy;
         const uint32_t bo = (uint32_t)j * zsb + ob;
@@ -681,9 +687,12 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       float cxx, cyy, cxy;
       if constexpr (CS == DCX_COEF_N16) {
         const float na = Cf::lo(a, 0, L.cxx), nb = Cf::hi(a, 1, L.cxx);
+        const float na2 = na + na;
         cxx = na * na;
         cyy = nb * nb;
-        cxy = (na + na) * nb;
+        cxy = na2 * nb;
+        Na2[u & 3] = na2;
+        Nb[u & 3] = nb;
       } else {
         cxx = Cf::dec(a, CXX, L.cxx);
         cyy = Cf::dec(a, CYY, L.cyy);

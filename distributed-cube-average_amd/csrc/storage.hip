@@ -262,9 +262,10 @@ __device__ __forceinline__ uint32_t axis_code(double v, float lo, float st) {
 }
 
 // Codes of every logical cell, and per block the largest error of the terms
-// the fast kernel rebuilds from them (its own float operations): absolute
-// for the six angle terms (out[2 block]), relative for the three v^2 terms
-// that are rebuilt (out[2 block + 1]).
+// the codes stand for: absolute for the six angle terms, formed by the fast
+// kernel's own float operations (out[2 block]), and relative for the three
+// v^2 terms as v2pz times the decoded ratio (out[2 block + 1]; the kernel
+// factors v2pz out of the right-hand sides instead of forming them).
 __global__ void __launch_bounds__(256)
     k_axis_encode(Coef10 cs, int64_t sx, int64_t ys, int64_t nrows,
                   AxisCode q, uint32_t *ab, uint64_t *cr, float *out) {
