@@ -320,13 +320,15 @@ template <> struct CoefIO<DCX_COEF_N16> {
       default: return 0u;
     }
   }
+  // Decode of the low or high code of r; off = c_lo[k], passed in from a
+  // VGPR (see fast_tile).
   static __device__ __forceinline__ float lo(const StepArgs &a, int k,
-                                             uint32_t r) {
-    return fmaf((float)(r & 0xffffu), a.c_st[k], a.c_lo[k]);
+                                             uint32_t r, float off) {
+    return fmaf((float)(r & 0xffffu), a.c_st[k], off);
   }
   static __device__ __forceinline__ float hi(const StepArgs &a, int k,
-                                             uint32_t r) {
-    return fmaf((float)(r >> 16), a.c_st[k], a.c_lo[k]);
+                                             uint32_t r, float off) {
+    return fmaf((float)(r >> 16), a.c_st[k], off);
   }
 };
 
@@ -452,6 +454,17 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   float Sp[4], Hp[4], Sq[4], Hq[4];
   // Axis form: 2a and b of the staged planes' cells, kept for their output.
   float Na2[4], Nb[4];
+  // Axis form: the six decode offsets, held in VGPRs for the whole tile.  A
+  // VOP3 instruction reads at most one SGPR, so a decode fma whose scale and
+  // offset both sat in SGPRs would first copy one of them into a VGPR, once
+  // per decode; the empty asm makes each offset a VGPR value the compiler
+  // cannot re-derive from its SGPR.
+  float clo[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    clo[k] = CS == DCX_COEF_N16 ? a.c_lo[k] : 0.0f;
+    if constexpr (CS == DCX_COEF_N16) asm volatile("" : "+v"(clo[k]));
+  }
   // Whether a 16-bit store of this thread saturated (or met a non-finite
   // value); counted once per wave in a.ovf at the end of the tile.
   bool ovf = false;
@@ -614,15 +627,16 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
           // v2pz times the ratios rx, rn, rs, so v2pz factors out:
           //   rhsp = v2pz (rx H2p + H1q + rs (H1p - H1q)),
           //   rhsq = v2pz (rn H2p + H1q - rs (H2p - H2q)).
-          const float nc = Cf::lo(a, 2, L.cxz);
+          const float nc = Cf::lo(a, 2, L.cxz, clo[2]);
           const float czz = nc * nc, cxz = nc * Na2[D], cyz = (nc + nc) * Nb[D];
           const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
           const float h2p = Sp[D] + pzz - h1p;
           const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
           const float h2q = Sq[D] + qzz - h1q;
           const float vpz = __uint_as_float(L.vpz);
-          const float rx = Cf::hi(a, 3, L.cxz), rn = Cf::lo(a, 4, L.cyz);
-          const float rs = Cf::hi(a, 5, L.cyz);
+          const float rx = Cf::hi(a, 3, L.cxz, clo[3]);
+          const float rn = Cf::lo(a, 4, L.cyz, clo[4]);
+          const float rs = Cf::hi(a, 5, L.cyz, clo[5]);
           rhsp = vpz * (rx * h2p + h1q + rs * (h1p - h1q));
           rhsq = vpz * (rn * h2p + h1q - rs * (h2p - h2q));
         } else {
@@ -686,7 +700,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       yq[u] = dyq;
       float cxx, cyy, cxy;
       if constexpr (CS == DCX_COEF_N16) {
-        const float na = Cf::lo(a, 0, L.cxx), nb = Cf::hi(a, 1, L.cxx);
+        const float na = Cf::lo(a, 0, L.cxx, clo[0]);
+        const float nb = Cf::hi(a, 1, L.cxx, clo[1]);
         const float na2 = na + na;
         cxx = na * na;
         cyy = nb * nb;
