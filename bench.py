@@ -140,7 +140,7 @@ def open_device(H, w, args):
     tx, ty, zc = args.tiling
     if not d.set_tiling(tx, ty, zc):
         raise SystemExit(f"unsupported tiling {args.tiling}")
-    d.set_xcd_remap(bool(args.xcd_remap))
+    d.set_xcd_remap(args.xcd_remap)
     d.set_prefetch_depth(args.prefetch)
     send, recv = octant_halo_boxes(w.sizes)
     d.set_halo_loopback(send, recv)

@@ -83,7 +83,7 @@ struct StepArgs {
   int *ovf;
   int fstore, cstore;              // DCX_STORE_*, DCX_COEF_*
   int zchunk;     // planes per workgroup (fast kernel)
-  int xcd_remap;  // give each XCD a contiguous run of tiles (fast kernel)
+  int xcd_remap;  // tile order over the XCDs: 0 round robin, 1 runs, 2 rows
   int deep;       // fast kernel prefetch depth - 1 (0, 1, 2)
   int gx, gy;     // tiles along x and y (set by the launcher)
 };

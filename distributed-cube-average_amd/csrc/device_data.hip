@@ -716,10 +716,10 @@ int dcx_set_prefetch_depth(dc_device_data *data, int planes) {
   return prev;
 }
 
-int dcx_set_xcd_remap(dc_device_data *data, int enabled) {
+int dcx_set_xcd_remap(dc_device_data *data, int mode) {
   State *st = state_of(data);
   const int prev = st->xcd_remap;
-  st->xcd_remap = enabled != 0;
+  st->xcd_remap = mode == 2 ? 2 : mode != 0;
   return prev;
 }
 

@@ -730,12 +730,20 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 
 template <int TX, int TY, int PF, int FS, int CS>
 __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
-  // Workgroups are dealt to the 8 XCDs round-robin by linear id; remapping
+  // Workgroups are dealt to the 8 XCDs round-robin by linear id.  Mode 1
   // gives each XCD (and its L2) a contiguous run of tiles, x fastest, so the
   // apron rows and columns a tile shares with its neighbours stay in one L2.
+  // Mode 2 gives XCD k the whole tile rows r = k mod 8, in the default row
+  // order: x neighbours share an L2, and the tiles in flight at once still
+  // cover one contiguous band of rows.
   int b = blockIdx.x;
   const int G = gridDim.x;
-  if (a.xcd_remap && G % 8 == 0) b = (b % 8) * (G / 8) + b / 8;
+  if (a.xcd_remap == 1 && G % 8 == 0) {
+    b = (b % 8) * (G / 8) + b / 8;
+  } else if (a.xcd_remap == 2 && G % a.gx == 0 && (G / a.gx) % 8 == 0) {
+    const int j = b / 8;
+    b = j % a.gx + a.gx * (b % 8 + 8 * (j / a.gx));
+  }
   fast_tile<TX, TY, PF, FS, CS>(a, b, a.src_amp);
 }
 

@@ -363,8 +363,10 @@ class DeviceData:
     def set_prefetch_depth(self, planes: int) -> int:
         return load().dcx_set_prefetch_depth(self.ptr, planes)
 
-    def set_xcd_remap(self, on: bool) -> bool:
-        return bool(load().dcx_set_xcd_remap(self.ptr, int(on)))
+    def set_xcd_remap(self, mode) -> int:
+        """dcx_set_xcd_remap: 0 off, 1 (or True) contiguous runs, 2 whole
+        tile rows per XCD; returns the previous mode."""
+        return int(load().dcx_set_xcd_remap(self.ptr, int(mode)))
 
     def set_storage(self, field_store: int, coef_store: int,
                     field_scale_log2: int = 8) -> bool:

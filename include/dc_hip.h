@@ -50,10 +50,15 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
  * Returns the previous depth. */
 int dcx_set_prefetch_depth(dc_device_data *data, int planes);
 
-/* Fast kernel: deal tiles to the GPU's 8 XCDs in contiguous runs so that
- * neighbouring tiles share an L2 (default off: measured slower on the
- * 512^3 block).  Returns the previous value. */
-int dcx_set_xcd_remap(dc_device_data *data, int enabled);
+/* Fast kernel: which XCD (and L2) runs which tile.  0 (default): the
+ * hardware's round robin over workgroup ids, so x-neighbour tiles run on
+ * different XCDs.  1: each XCD takes a contiguous run of tiles.  2: XCD k
+ * takes the whole tile rows r = k mod 8 (when the row count is a multiple
+ * of 8), so x neighbours share an L2 while the rows in flight stay one
+ * band.  On the 512^3 block modes 1 and 2 are 7-13% slower for FP32 fields
+ * and coefficients, and 2-6% faster for the A16 and FP16 forms; results are
+ * the same bits in every mode.  Returns the previous mode. */
+int dcx_set_xcd_remap(dc_device_data *data, int mode);
 
 /* Wait for all queued work of this struct. */
 void dcx_synchronize(dc_device_data *data);
