@@ -125,6 +125,14 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
     a.cf[CXX] = ab;
     a.cf[CXZ] = a.cf[CYZ] = cr;
     a.cf[V2PZ] = a.coef.c[V2PZ];
+    // The kernel decodes the axis over the spacings, (n_x/dx, n_y/dy,
+    // n_z/dz): products of two components then carry the 1/(d_a d_b) of
+    // their derivative (x2 off the diagonal, as the angle terms have it).
+    const double inv[3] = {1.0 / dx, 1.0 / dy, 1.0 / dz};
+    for (int k = 0; k < 3; ++k) {
+      a.c_lo[k] = (float)(st->c_lo[k] * inv[k]);
+      a.c_st[k] = (float)(st->c_st[k] * inv[k]);
+    }
   }
   return a;
 }

@@ -355,6 +355,12 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   using F = FieldIO<FS>;
   using Cf = CoefIO<CS>;
   constexpr bool kTimeDiff = FS != DCX_STORE_F32;
+  constexpr bool kAxis = CS == DCX_COEF_N16;
+  // Spacing factors of the derivatives.  The axis form leaves them out of
+  // the in-plane and mixed derivatives (x 1 folds away): its decoded axis is
+  // (n_x/dx, n_y/dy, n_z/dz), set per launch, so the products of two
+  // components carry them; only the Laplacian's sum scales (Sp, H2).
+  auto sp = [](float d) { return kAxis ? 1.0f : d; };
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 8;       // LDS plane row pitch (4-cell apron)
   constexpr int PH = TY + 8;
@@ -574,8 +580,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       // In-plane derivatives of plane n.
       const Pairs pxq = lds_pairs(sP, lo, 1), pyq = lds_pairs(sP, lo, PW);
       const Pairs qxq = lds_pairs(sQ, lo, 1), qyq = lds_pairs(sQ, lo, PW);
-      const float pxx = d2_of(cp, pxq) * a.dxx, pyy = d2_of(cp, pyq) * a.dyy;
-      const float qxx = d2_of(cq, qxq) * a.dxx, qyy = d2_of(cq, qyq) * a.dyy;
+      const float pxx = d2_of(cp, pxq) * sp(a.dxx), pyy = d2_of(cp, pyq) * sp(a.dyy);
+      const float qxx = d2_of(cq, qxq) * sp(a.dxx), qyy = d2_of(cq, qyq) * sp(a.dyy);
       const float dxp = d1_of(pxq), dyp = d1_of(pyq);
       const float dxq = d1_of(qxq), dyq = d1_of(qyq);
       sDP[ld] = dxp;
@@ -586,8 +592,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         sDQ[dr * TX + dc] = d1_lds(sQ, l, 1);
       }
       __syncthreads();
-      const float pxy = d1_lds(sDP, ld, TX) * a.dxy;
-      const float qxy = d1_lds(sDQ, ld, TX) * a.dxy;
+      const float pxy = d1_lds(sDP, ld, TX) * sp(a.dxy);
+      const float qxy = d1_lds(sDQ, ld, TX) * sp(a.dxy);
 
       // Output plane j = n - 4 once the rings hold planes j-4 .. j+4.
       if (out) {
@@ -597,48 +603,51 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         const float pzz = (W_K0 * rp[J] + W_K1 * (RS(rp, 1) + RS(rp, 7)) +
                            W_K2 * (RS(rp, 2) + RS(rp, 6)) +
                            W_K3 * (RS(rp, 3) + RS(rp, 5)) +
-                           W_K4 * (cp + rp[O])) * a.dzz;
+                           W_K4 * (cp + rp[O])) * sp(a.dzz);
         const float qzz = (W_K0 * rq[J] + W_K1 * (RS(rq, 1) + RS(rq, 7)) +
                            W_K2 * (RS(rq, 2) + RS(rq, 6)) +
                            W_K3 * (RS(rq, 3) + RS(rq, 5)) +
-                           W_K4 * (cq + rq[O])) * a.dzz;
+                           W_K4 * (cq + rq[O])) * sp(a.dzz);
         const float pxz = (W_L1 * (RS(xp, 1) - RS(xp, 7)) +
                            W_L2 * (RS(xp, 2) - RS(xp, 6)) +
                            W_L3 * (RS(xp, 3) - RS(xp, 5)) +
-                           W_L4 * (dxp - xp[O])) * a.dxz;
+                           W_L4 * (dxp - xp[O])) * sp(a.dxz);
         const float pyz = (W_L1 * (RS(yp, 1) - RS(yp, 7)) +
                            W_L2 * (RS(yp, 2) - RS(yp, 6)) +
                            W_L3 * (RS(yp, 3) - RS(yp, 5)) +
-                           W_L4 * (dyp - yp[O])) * a.dyz;
+                           W_L4 * (dyp - yp[O])) * sp(a.dyz);
         const float qxz = (W_L1 * (RS(xq, 1) - RS(xq, 7)) +
                            W_L2 * (RS(xq, 2) - RS(xq, 6)) +
                            W_L3 * (RS(xq, 3) - RS(xq, 5)) +
-                           W_L4 * (dxq - xq[O])) * a.dxz;
+                           W_L4 * (dxq - xq[O])) * sp(a.dxz);
         const float qyz = (W_L1 * (RS(yq, 1) - RS(yq, 7)) +
                            W_L2 * (RS(yq, 2) - RS(yq, 6)) +
                            W_L3 * (RS(yq, 3) - RS(yq, 5)) +
-                           W_L4 * (dyq - yq[O])) * a.dyz;
+                           W_L4 * (dyq - yq[O])) * sp(a.dyz);
 #undef RS
         const int D = u & 3;
         float rhsp, rhsq;
-        if constexpr (CS == DCX_COEF_N16) {
-          // n = (na, nb, nc): czz = nc^2, cxz = 2 na nc, cyz = 2 nb nc, with
-          // 2na and nb kept from the plane's staging.  v2px, v2pn, v2sz are
-          // v2pz times the ratios rx, rn, rs, so v2pz factors out:
+        // Axis form: v2pz dt^2, the factor of both right-hand sides.
+        float vdt = 0.0f;
+        if constexpr (kAxis) {
+          // n = (na, nb, nc) over the spacings: czz = nc^2, cxz = 2 na nc,
+          // cyz = 2 nb nc, with 2na and nb kept from the plane's staging.
+          // v2px, v2pn, v2sz are v2pz times the ratios rx, rn, rs, so v2pz
+          // factors out (and is applied with dt^2 in the update):
           //   rhsp = v2pz (rx H2p + H1q + rs (H1p - H1q)),
           //   rhsq = v2pz (rn H2p + H1q - rs (H2p - H2q)).
           const float nc = Cf::lo(a, 2, L.cxz, clo[2]);
           const float czz = nc * nc, cxz = nc * Na2[D], cyz = (nc + nc) * Nb[D];
           const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
-          const float h2p = Sp[D] + pzz - h1p;
+          const float h2p = Sp[D] + pzz * a.dzz - h1p;
           const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
-          const float h2q = Sq[D] + qzz - h1q;
-          const float vpz = __uint_as_float(L.vpz);
+          const float h2q = Sq[D] + qzz * a.dzz - h1q;
+          vdt = __uint_as_float(L.vpz) * dt2;
           const float rx = Cf::hi(a, 3, L.cxz, clo[3]);
           const float rn = Cf::lo(a, 4, L.cyz, clo[4]);
           const float rs = Cf::hi(a, 5, L.cyz, clo[5]);
-          rhsp = vpz * (rx * h2p + h1q + rs * (h1p - h1q));
-          rhsq = vpz * (rn * h2p + h1q - rs * (h2p - h2q));
+          rhsp = rx * h2p + h1q + rs * (h1p - h1q);
+          rhsq = rn * h2p + h1q - rs * (h2p - h2q);
         } else {
           const float czz = Cf::dec(a, CZZ, L.czz);
           const float cxz = Cf::dec(a, CXZ, L.cxz);
@@ -678,9 +687,16 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
           }
         } else {
           // (rhs * dt) * dt, the reference's association
-          // (include/sample_compute.h:181-184).
-          float pn = 2.0f * rp[J] - opp + rhsp * a.dt * a.dt;
-          float qn = 2.0f * rq[J] - oqp + rhsq * a.dt * a.dt;
+          // (include/sample_compute.h:181-184); the axis form multiplies its
+          // rhs by v2pz dt^2 instead.
+          float pn, qn;
+          if constexpr (kAxis) {
+            pn = 2.0f * rp[J] - opp + rhsp * vdt;
+            qn = 2.0f * rq[J] - oqp + rhsq * vdt;
+          } else {
+            pn = 2.0f * rp[J] - opp + rhsp * a.dt * a.dt;
+            qn = 2.0f * rq[J] - oqp + rhsq * a.dt * a.dt;
+          }
           if (src) {
             pn += src_amp;
             qn += src_amp;
@@ -713,9 +729,14 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         cyy = Cf::dec(a, CYY, L.cyy);
         cxy = Cf::dec(a, CXY, L.cxy);
       }
-      Sp[u & 3] = pxx + pyy;
+      if constexpr (kAxis) {
+        Sp[u & 3] = pxx * a.dxx + pyy * a.dyy;
+        Sq[u & 3] = qxx * a.dxx + qyy * a.dyy;
+      } else {
+        Sp[u & 3] = pxx + pyy;
+        Sq[u & 3] = qxx + qyy;
+      }
       Hp[u & 3] = cxx * pxx + cyy * pyy + cxy * pxy;
-      Sq[u & 3] = qxx + qyy;
       Hq[u & 3] = cxx * qxx + cyy * qyy + cxy * qxy;
 
       cur = nxt;

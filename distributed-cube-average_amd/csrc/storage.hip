@@ -655,6 +655,43 @@ extern "C" int dcx_storage_overflow(dc_device_data *data) {
   return storage_overflowed(state_of(data)) ? 1 : 0;
 }
 
+namespace dcx {
+namespace {
+// Zero n 32-bit words (grid-stride; a fill kernel rather than a memset node,
+// so the call can sit in a captured graph).
+__global__ void __launch_bounds__(256) k_zero_words(uint32_t *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// `bytes` is a multiple of 4: the arrays hold alloc_floats (a multiple of 64)
+// elements of 4 or 2 bytes, the overflow flag is one int.
+void zero_bytes(State *st, void *p, size_t bytes) {
+  hipLaunchKernelGGL(k_zero_words, dim3(kConvBlocks), dim3(256), 0, st->stream,
+                     static_cast<uint32_t *>(p), (int64_t)(bytes / 4));
+  DCX_CHECK(hipGetLastError());
+}
+}  // namespace
+}  // namespace dcx
+
+extern "C" void dcx_zero_fields(dc_device_data *data) {
+  State *st = state_of(data);
+  const size_t n = (size_t)st->alloc_floats;
+  if (st->fstore == DCX_STORE_F32) {
+    for (int f = 0; f < 4; ++f) zero_bytes(st, st->field_alloc[f], n * 4);
+  } else {
+    for (int f = 0; f < 2; ++f) {
+      zero_bytes(st, whole(st->P16[f][0]), n * 2);
+      zero_bytes(st, whole(st->P16[f][1]), n * 2);
+      zero_bytes(st, whole(st->D16[f]), n * 2);
+    }
+    // Fields at rest are in range again: clear the overflow flag.
+    zero_bytes(st, st->ovf, sizeof(int));
+    st->ovf_reported = false;
+  }
+}
+
 extern "C" int dcx_bytes_per_cell(const dc_device_data *data) {
   const State *st = state_of(data);
   const int fb = st->fstore == DCX_STORE_F32 ? 4 : 2;

@@ -99,6 +99,7 @@ SIGNATURES = {
     "dcx_set_storage": (C.c_int, [_DD, C.c_int, C.c_int, C.c_int]),
     "dcx_bytes_per_cell": (C.c_int, [_DD]),
     "dcx_storage_overflow": (C.c_int, [_DD]),
+    "dcx_zero_fields": (None, [_DD]),
     "dcx_download": (None, [_DD, C.c_int, _FP]),
     "dcx_upload": (None, [_DD, C.c_int, _FP]),
     "dcx_step": (None, [_DD, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -378,6 +379,11 @@ class DeviceData:
         """dcx_storage_overflow: whether a 16-bit field store has saturated
         since the last conversion."""
         return bool(load().dcx_storage_overflow(self.ptr))
+
+    def zero_fields(self):
+        """dcx_zero_fields: both time levels of p and q back to rest, in the
+        current storage form (half storage keeps its scales)."""
+        load().dcx_zero_fields(self.ptr)
 
     def bytes_per_cell(self) -> int:
         return load().dcx_bytes_per_cell(self.ptr)

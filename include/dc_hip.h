@@ -125,6 +125,12 @@ int dcx_set_storage(dc_device_data *data, int field_store, int coef_store,
  * ~1500 steps on the octant model), so long runs do reach the range. */
 int dcx_storage_overflow(dc_device_data *data);
 
+/* Set both time levels of p and q to zero (a run from rest) in the struct's
+ * current storage form, on its stream with a fill kernel (no memset node, so
+ * it may be captured in a graph).  Half storage keeps its scales, and its
+ * overflow flag is cleared.  Coefficients are untouched. */
+void dcx_zero_fields(dc_device_data *data);
+
 /* Bytes per cell-update the fast kernel moves (algorithmic, no apron) for
  * the struct's current storage. */
 int dcx_bytes_per_cell(const dc_device_data *data);

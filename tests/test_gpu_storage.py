@@ -183,3 +183,46 @@ def test_half_storage_scale_follows_the_fields(H, run40):
     for k in (1, 3):
         assert np.isfinite(after[k]).all()
         assert np.abs(after[k] - before[k]).max() <= 2e-3 * big
+
+
+@pytest.mark.parametrize("form", [(0, 0), (0, 4), (1, 1)])
+def test_zero_fields_is_a_run_from_rest(H, run40, form):
+    """dcx_zero_fields after 20 steps, then 15 steps with the source: the
+    bits of a fresh struct of the same form stepped 15 times (the scales a
+    half form picked at conversion from zero fields are kept)."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    out = []
+    for warm in (20, 0):
+        dd = H.DeviceData(coef, None, d, d, d, dt)
+        assert dd.set_storage(*form, 8)
+        if warm:
+            dd.run(warm, src, amp[:warm])
+            dd.zero_fields()
+            for f in dd.fields():
+                assert not f.any()
+        dd.run(15, src, amp[:15], graph=True)
+        out.append(dd.fields())
+        dd.close()
+    assert np.abs(out[1][1]).max() > 0
+    for a, b in zip(*out):
+        assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_zero_fields_clears_the_overflow_flag(H, run40, capfd):
+    """A half form that saturated (source far beyond its range) is in range
+    again after dcx_zero_fields: the flag is clear, and a later saturation is
+    reported again."""
+    coef, dt, steps, src, d, amp, f64 = run40
+    dd = H.DeviceData(coef, None, d, d, d, dt)
+    assert dd.set_storage(1, 1, 8)
+    big = (1e5 * amp).astype(np.float32)
+    dd.run(steps, src, big)
+    assert dd.storage_overflow()
+    dd.zero_fields()
+    assert not dd.storage_overflow()
+    dd.run(steps, src, big)
+    dd.synchronize()
+    flagged = dd.storage_overflow()
+    dd.close()
+    assert flagged
+    assert capfd.readouterr().err.count("field storage left its range") == 2
