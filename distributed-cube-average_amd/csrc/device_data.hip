@@ -2,6 +2,7 @@
 // one MI355X, plus the library's own dcx_* entry points for device-resident
 // time loops.  All work of one dc_device_data is ordered on one HIP stream.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -125,10 +126,12 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
     a.cf[CXX] = ab;
     a.cf[CXZ] = a.cf[CYZ] = cr;
     a.cf[V2PZ] = a.coef.c[V2PZ];
-    // The kernel decodes the axis over the spacings, (n_x/dx, n_y/dy,
-    // n_z/dz): products of two components then carry the 1/(d_a d_b) of
-    // their derivative (x2 off the diagonal, as the angle terms have it).
-    const double inv[3] = {1.0 / dx, 1.0 / dy, 1.0 / dz};
+    // The kernel decodes the axis as sqrt(2) (n_x/dx, n_y/dy, n_z/dz): a
+    // product of two components is then the angle term over d_a d_b (2 n_a
+    // n_b off the diagonal; twice n_a^2 on it, which the kernel's
+    // half-weight second differences undo).
+    const double inv[3] = {std::sqrt(2.0) / dx, std::sqrt(2.0) / dy,
+                           std::sqrt(2.0) / dz};
     for (int k = 0; k < 3; ++k) {
       a.c_lo[k] = (float)(st->c_lo[k] * inv[k]);
       a.c_st[k] = (float)(st->c_st[k] * inv[k]);

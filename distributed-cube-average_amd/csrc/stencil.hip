@@ -152,8 +152,10 @@ __device__ __forceinline__ Pairs lds_pairs(const float *L, int l, int st) {
                p1 - m1, p2 - m2, p3 - m3, p4 - m4};
 }
 
-__device__ __forceinline__ float d2_of(float c, const Pairs &q) {
-  return W_K0 * c + W_K1 * q.s1 + W_K2 * q.s2 + W_K3 * q.s3 + W_K4 * q.s4;
+// h scales the weights (exactly: 1 or 1/2).
+__device__ __forceinline__ float d2_of(float c, const Pairs &q, float h) {
+  return (h * W_K0) * c + (h * W_K1) * q.s1 + (h * W_K2) * q.s2 +
+         (h * W_K3) * q.s3 + (h * W_K4) * q.s4;
 }
 
 __device__ __forceinline__ float d1_of(const Pairs &q) {
@@ -358,9 +360,12 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   constexpr bool kAxis = CS == DCX_COEF_N16;
   // Spacing factors of the derivatives.  The axis form leaves them out of
   // the in-plane and mixed derivatives (x 1 folds away): its decoded axis is
-  // (n_x/dx, n_y/dy, n_z/dz), set per launch, so the products of two
-  // components carry them; only the Laplacian's sum scales (Sp, H2).
+  // sqrt(2) (n_x/dx, n_y/dy, n_z/dz), set per launch, so the products of two
+  // components are the angle terms over d_a d_b (the squares twice theirs);
+  // only the Laplacian's sum scales (Sp, H2).  Its second differences take
+  // half weights (kh) to undo the doubled squares.
   auto sp = [](float d) { return kAxis ? 1.0f : d; };
+  constexpr float kh = kAxis ? 0.5f : 1.0f;
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 8;       // LDS plane row pitch (4-cell apron)
   constexpr int PH = TY + 8;
@@ -458,8 +463,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // Register rings (slot = plane index - (zb-4), mod 8 / mod 4).
   float rp[8], rq[8], xp[8], yp[8], xq[8], yq[8];
   float Sp[4], Hp[4], Sq[4], Hq[4];
-  // Axis form: 2a and b of the staged planes' cells, kept for their output.
-  float Na2[4], Nb[4];
+  // Axis form: a and b of the staged planes' cells, kept for their output.
+  float Na[4], Nb[4];
   // Axis form: the six decode offsets, held in VGPRs for the whole tile.  A
   // VOP3 instruction reads at most one SGPR, so a decode fma whose scale and
   // offset both sat in SGPRs would first copy one of them into a VGPR, once
@@ -477,7 +482,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
 #pragma unroll
   for (int k = 0; k < 8; ++k) rp[k] = rq[k] = xp[k] = yp[k] = xq[k] = yq[k] = 0.f;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) Sp[k] = Hp[k] = Sq[k] = Hq[k] = Na2[k] = Nb[k] = 0.f;
+  for (int k = 0; k < 4; ++k) Sp[k] = Hp[k] = Sq[k] = Hq[k] = Na[k] = Nb[k] = 0.f;
 
   const int nend = ze + 4;  // planes zb-4 .. ze+3 are staged
   auto load_staged = [&](int k) {
@@ -580,8 +585,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       // In-plane derivatives of plane n.
       const Pairs pxq = lds_pairs(sP, lo, 1), pyq = lds_pairs(sP, lo, PW);
       const Pairs qxq = lds_pairs(sQ, lo, 1), qyq = lds_pairs(sQ, lo, PW);
-      const float pxx = d2_of(cp, pxq) * sp(a.dxx), pyy = d2_of(cp, pyq) * sp(a.dyy);
-      const float qxx = d2_of(cq, qxq) * sp(a.dxx), qyy = d2_of(cq, qyq) * sp(a.dyy);
+      const float pxx = d2_of(cp, pxq, kh) * sp(a.dxx);
+      const float pyy = d2_of(cp, pyq, kh) * sp(a.dyy);
+      const float qxx = d2_of(cq, qxq, kh) * sp(a.dxx);
+      const float qyy = d2_of(cq, qyq, kh) * sp(a.dyy);
       const float dxp = d1_of(pxq), dyp = d1_of(pyq);
       const float dxq = d1_of(qxq), dyq = d1_of(qyq);
       sDP[ld] = dxp;
@@ -600,14 +607,16 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         const int J = (u + 4) & 7;  // slot of plane j
         const int O = u;            // slot of plane j-4 (about to be replaced)
 #define RS(r, b) r[(J + (b)) & 7]
-        const float pzz = (W_K0 * rp[J] + W_K1 * (RS(rp, 1) + RS(rp, 7)) +
-                           W_K2 * (RS(rp, 2) + RS(rp, 6)) +
-                           W_K3 * (RS(rp, 3) + RS(rp, 5)) +
-                           W_K4 * (cp + rp[O])) * sp(a.dzz);
-        const float qzz = (W_K0 * rq[J] + W_K1 * (RS(rq, 1) + RS(rq, 7)) +
-                           W_K2 * (RS(rq, 2) + RS(rq, 6)) +
-                           W_K3 * (RS(rq, 3) + RS(rq, 5)) +
-                           W_K4 * (cq + rq[O])) * sp(a.dzz);
+        const float pzz = ((kh * W_K0) * rp[J] +
+                           (kh * W_K1) * (RS(rp, 1) + RS(rp, 7)) +
+                           (kh * W_K2) * (RS(rp, 2) + RS(rp, 6)) +
+                           (kh * W_K3) * (RS(rp, 3) + RS(rp, 5)) +
+                           (kh * W_K4) * (cp + rp[O])) * sp(a.dzz);
+        const float qzz = ((kh * W_K0) * rq[J] +
+                           (kh * W_K1) * (RS(rq, 1) + RS(rq, 7)) +
+                           (kh * W_K2) * (RS(rq, 2) + RS(rq, 6)) +
+                           (kh * W_K3) * (RS(rq, 3) + RS(rq, 5)) +
+                           (kh * W_K4) * (cq + rq[O])) * sp(a.dzz);
         const float pxz = (W_L1 * (RS(xp, 1) - RS(xp, 7)) +
                            W_L2 * (RS(xp, 2) - RS(xp, 6)) +
                            W_L3 * (RS(xp, 3) - RS(xp, 5)) +
@@ -630,18 +639,20 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         // Axis form: v2pz dt^2, the factor of both right-hand sides.
         float vdt = 0.0f;
         if constexpr (kAxis) {
-          // n = (na, nb, nc) over the spacings: czz = nc^2, cxz = 2 na nc,
-          // cyz = 2 nb nc, with 2na and nb kept from the plane's staging.
-          // v2px, v2pn, v2sz are v2pz times the ratios rx, rn, rs, so v2pz
-          // factors out (and is applied with dt^2 in the update):
+          // n = (na, nb, nc) as decoded (above): czz = nc^2 with the
+          // half-weight pzz, cxz = na nc, cyz = nb nc, with na and nb kept
+          // from the plane's staging.  Sp and pzz dzz are half the
+          // Laplacian's terms.  v2px, v2pn, v2sz are v2pz times the ratios
+          // rx, rn, rs, so v2pz factors out (and is applied with dt^2 in the
+          // update):
           //   rhsp = v2pz (rx H2p + H1q + rs (H1p - H1q)),
           //   rhsq = v2pz (rn H2p + H1q - rs (H2p - H2q)).
           const float nc = Cf::lo(a, 2, L.cxz, clo[2]);
-          const float czz = nc * nc, cxz = nc * Na2[D], cyz = (nc + nc) * Nb[D];
+          const float czz = nc * nc, cxz = nc * Na[D], cyz = nc * Nb[D];
           const float h1p = Hp[D] + czz * pzz + cxz * pxz + cyz * pyz;
-          const float h2p = Sp[D] + pzz * a.dzz - h1p;
+          const float h2p = 2.0f * (Sp[D] + pzz * a.dzz) - h1p;
           const float h1q = Hq[D] + czz * qzz + cxz * qxz + cyz * qyz;
-          const float h2q = Sq[D] + qzz * a.dzz - h1q;
+          const float h2q = 2.0f * (Sq[D] + qzz * a.dzz) - h1q;
           vdt = __uint_as_float(L.vpz) * dt2;
           const float rx = Cf::hi(a, 3, L.cxz, clo[3]);
           const float rn = Cf::lo(a, 4, L.cyz, clo[4]);
@@ -665,9 +676,9 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         }
         const float opp = F::dec(L.opp), oqp = F::dec(L.oqp);
         const int j = n - 4;
-        const bool src = j == a.src_z && (int64_t)own == a.src_xy;
         const uint32_t bo = (uint32_t)j * zsb + ob;
         if (kTimeDiff) {
+          const bool src = j == a.src_z && (int64_t)own == a.src_xy;
           // D_new = D + dt^2 rhs; P_new = P + D_new; the next step's source
           // raises P_new, and with it D_new, by src_amp (field units).
           float dnp = opp + rhsp * dt2d, dnq = oqp + rhsq * dt2d;
@@ -697,10 +708,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
             pn = 2.0f * rp[J] - opp + rhsp * a.dt * a.dt;
             qn = 2.0f * rq[J] - oqp + rhsq * a.dt * a.dt;
           }
-          if (src) {
-            pn += src_amp;
-            qn += src_amp;
-          }
+          // The source is added after the tile (below).
           if (act) {
             F::st(Gpp, bo, pn);
             F::st(Gqp, bo, qn);
@@ -718,11 +726,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       if constexpr (CS == DCX_COEF_N16) {
         const float na = Cf::lo(a, 0, L.cxx, clo[0]);
         const float nb = Cf::hi(a, 1, L.cxx, clo[1]);
-        const float na2 = na + na;
         cxx = na * na;
         cyy = nb * nb;
-        cxy = na2 * nb;
-        Na2[u & 3] = na2;
+        cxy = na * nb;
+        Na[u & 3] = na;
         Nb[u & 3] = nb;
       } else {
         cxx = Cf::dec(a, CXX, L.cxx);
@@ -746,6 +753,16 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // set; the flag stays set until the storage is converted again.
   if constexpr (kTimeDiff) {
     if (ovf && a.ovf != nullptr) atomicOr(a.ovf, 1);
+  } else {
+    // The next step's source, added by the lane that stored the source cell
+    // to the value it stored: the same two roundings as adding it before
+    // the store (the reference's add-then-swap, src/device_data.c:44-59),
+    // with no per-plane test.  No other lane reads the new level's cells.
+    if (act && a.src_z >= zb && a.src_z < ze && (int64_t)own == a.src_xy) {
+      const uint32_t bs = (uint32_t)a.src_z * zsb + ob;
+      F::st(Gpp, bs, F::ld(Gpp, bs) + src_amp);
+      F::st(Gqp, bs, F::ld(Gqp, bs) + src_amp);
+    }
   }
 }
 
