@@ -359,12 +359,13 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   constexpr bool kTimeDiff = FS != DCX_STORE_F32;
   constexpr bool kAxis = CS == DCX_COEF_N16;
   // Spacing factors of the derivatives.  The axis form leaves them out of
-  // the in-plane and mixed derivatives (x 1 folds away): its decoded axis is
-  // sqrt(2) (n_x/dx, n_y/dy, n_z/dz), set per launch, so the products of two
-  // components are the angle terms over d_a d_b (the squares twice theirs);
-  // only the Laplacian's sum scales (Sp, H2).  Its second differences take
-  // half weights (kh) to undo the doubled squares.
-  auto sp = [](float d) { return kAxis ? 1.0f : d; };
+  // the in-plane and mixed derivatives (the factor becomes 1, which the
+  // compiler drops): its decoded axis is sqrt(2) (n_x/dx, n_y/dy, n_z/dz),
+  // set per launch, so the products of two components are the angle terms
+  // over d_a d_b (the squares twice theirs); only the Laplacian's sum scales
+  // (Sp, H2).  Its second differences take half weights (kh) to undo the
+  // doubled squares.
+  auto spf = [](float d) { return kAxis ? 1.0f : d; };
   constexpr float kh = kAxis ? 0.5f : 1.0f;
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 8;       // LDS plane row pitch (4-cell apron)
@@ -585,10 +586,10 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
       // In-plane derivatives of plane n.
       const Pairs pxq = lds_pairs(sP, lo, 1), pyq = lds_pairs(sP, lo, PW);
       const Pairs qxq = lds_pairs(sQ, lo, 1), qyq = lds_pairs(sQ, lo, PW);
-      const float pxx = d2_of(cp, pxq, kh) * sp(a.dxx);
-      const float pyy = d2_of(cp, pyq, kh) * sp(a.dyy);
-      const float qxx = d2_of(cq, qxq, kh) * sp(a.dxx);
-      const float qyy = d2_of(cq, qyq, kh) * sp(a.dyy);
+      const float pxx = d2_of(cp, pxq, kh) * spf(a.dxx);
+      const float pyy = d2_of(cp, pyq, kh) * spf(a.dyy);
+      const float qxx = d2_of(cq, qxq, kh) * spf(a.dxx);
+      const float qyy = d2_of(cq, qyq, kh) * spf(a.dyy);
       const float dxp = d1_of(pxq), dyp = d1_of(pyq);
       const float dxq = d1_of(qxq), dyq = d1_of(qyq);
       sDP[ld] = dxp;
@@ -599,8 +600,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         sDQ[dr * TX + dc] = d1_lds(sQ, l, 1);
       }
       __syncthreads();
-      const float pxy = d1_lds(sDP, ld, TX) * sp(a.dxy);
-      const float qxy = d1_lds(sDQ, ld, TX) * sp(a.dxy);
+      const float pxy = d1_lds(sDP, ld, TX) * spf(a.dxy);
+      const float qxy = d1_lds(sDQ, ld, TX) * spf(a.dxy);
 
       // Output plane j = n - 4 once the rings hold planes j-4 .. j+4.
       if (out) {
@@ -611,28 +612,28 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
                            (kh * W_K1) * (RS(rp, 1) + RS(rp, 7)) +
                            (kh * W_K2) * (RS(rp, 2) + RS(rp, 6)) +
                            (kh * W_K3) * (RS(rp, 3) + RS(rp, 5)) +
-                           (kh * W_K4) * (cp + rp[O])) * sp(a.dzz);
+                           (kh * W_K4) * (cp + rp[O])) * spf(a.dzz);
         const float qzz = ((kh * W_K0) * rq[J] +
                            (kh * W_K1) * (RS(rq, 1) + RS(rq, 7)) +
                            (kh * W_K2) * (RS(rq, 2) + RS(rq, 6)) +
                            (kh * W_K3) * (RS(rq, 3) + RS(rq, 5)) +
-                           (kh * W_K4) * (cq + rq[O])) * sp(a.dzz);
+                           (kh * W_K4) * (cq + rq[O])) * spf(a.dzz);
         const float pxz = (W_L1 * (RS(xp, 1) - RS(xp, 7)) +
                            W_L2 * (RS(xp, 2) - RS(xp, 6)) +
                            W_L3 * (RS(xp, 3) - RS(xp, 5)) +
-                           W_L4 * (dxp - xp[O])) * sp(a.dxz);
+                           W_L4 * (dxp - xp[O])) * spf(a.dxz);
         const float pyz = (W_L1 * (RS(yp, 1) - RS(yp, 7)) +
                            W_L2 * (RS(yp, 2) - RS(yp, 6)) +
                            W_L3 * (RS(yp, 3) - RS(yp, 5)) +
-                           W_L4 * (dyp - yp[O])) * sp(a.dyz);
+                           W_L4 * (dyp - yp[O])) * spf(a.dyz);
         const float qxz = (W_L1 * (RS(xq, 1) - RS(xq, 7)) +
                            W_L2 * (RS(xq, 2) - RS(xq, 6)) +
                            W_L3 * (RS(xq, 3) - RS(xq, 5)) +
-                           W_L4 * (dxq - xq[O])) * sp(a.dxz);
+                           W_L4 * (dxq - xq[O])) * spf(a.dxz);
         const float qyz = (W_L1 * (RS(yq, 1) - RS(yq, 7)) +
                            W_L2 * (RS(yq, 2) - RS(yq, 6)) +
                            W_L3 * (RS(yq, 3) - RS(yq, 5)) +
-                           W_L4 * (dyq - yq[O])) * sp(a.dyz);
+                           W_L4 * (dyq - yq[O])) * spf(a.dyz);
 #undef RS
         const int D = u & 3;
         float rhsp, rhsq;
