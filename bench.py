@@ -310,9 +310,10 @@ def precision_study(H, VM, steps_list, log_prefix="[bench]"):
 
 
 BATCH_FORMS = [("fp32", 0, 0), ("fp32_n16coef", 0, 4), ("fp16", 1, 1)]
-# Half field scale of the survey's shots, which start from rest: 2^4 keeps
-# the largest field (a 10 Hz wavelet builds up a few hundred units at its
-# source cell) 16x below half's largest finite value.
+# Half field scale of the survey's shots, which start from rest.  In FP32
+# the lowest-frequency shot (10 Hz) reaches 130 field units at its source
+# cell, half the 256 that 2^8 leaves, and the half form's rounding grows the
+# field further (one shot saturated at 2^8); 2^4 leaves 16x more room.
 BATCH_FIELD_SCALE_LOG2 = 4
 
 
