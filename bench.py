@@ -402,6 +402,71 @@ def batch_study(H, VM, args):
     return out
 
 
+def survey_shots(H, VM, args, fs, cs):
+    """The batch study's survey on the device in one storage form, fields at
+    rest: (survey, shots, source cells, source tables)."""
+    S = VM.layered_survey(seed=args.seed, n_shots=args.batch_shots,
+                          steps=args.batch_steps)
+    ds, idx, amps = [], [], []
+    for k in range(len(S)):
+        w = S.shot(k)
+        d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
+        d.release_host()
+        d.set_storage(fs, cs, BATCH_FIELD_SCALE_LOG2)
+        ds.append(d)
+        idx.append(w.src_index)
+        amps.append(w.source_table())
+    return S, ds, idx, np.stack(amps)
+
+
+def batch_sweep(H, VM, args):
+    """Every fixed batch tiling (z chunk, prefetch depth, workgroups per CU;
+    0 = one per tile) on the survey, FP32 and the axis form, 40 steps each
+    from rest: the space the autotuner searches."""
+    out = {}
+    for name, fs, cs in (("fp32", 0, 0), ("fp32_n16coef", 0, 4)):
+        S, ds, idx, amp = survey_shots(H, VM, args, fs, cs)
+        cells = S.compute_cells()
+        b = H.Batch(ds)
+        b.set_autotune(False)
+        rows = []
+        for zc in (128, 256, 512):
+            for pf in (1, 2, 3):
+                if pf == 3 and cs != 0:
+                    continue  # the three-plane prefetch is built for FP32 only
+                for bpc in (0, 1, 2, 4, 8):
+                    for d in ds:
+                        d.zero_fields()
+                    b.set_tiling(zc, pf, bpc)
+                    b.run(40, idx, amp, graph=True)
+                    v = round(cells / b.last_step_ms() / 1e3, 1)
+                    rows.append({"z_chunk": zc, "prefetch_planes": pf,
+                                 "workgroups_per_cu": bpc, "mcell_per_s": v})
+                    log(f"[bench] sweep {name} z {zc} prefetch {pf} "
+                        f"wg/cu {bpc}: {v:.0f} Mcell/s")
+        b.close()
+        for d in ds:
+            d.close()
+        out[name] = sorted(rows, key=lambda r: -r["mcell_per_s"])
+    return out
+
+
+def batch_child(args):
+    """Run under rocprofv3: the survey batched in one storage form with one
+    fixed tiling, --steps steps (no graph, so each step is a dispatch)."""
+    pkg = importlib.import_module("distributed-cube-average_amd")
+    H = pkg.dc_hip
+    VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
+    fs, cs, zc, pf, bpc = args.child_batch
+    S, ds, idx, amp = survey_shots(H, VM, args, fs, cs)
+    b = H.Batch(ds)
+    b.set_autotune(False)
+    b.set_tiling(zc, pf, bpc)
+    ms = b.run(args.steps, idx, amp, graph=False)
+    log(f"[bench] batch child: {ms / args.steps:.3f} ms per step")
+    b.close()
+
+
 def learned_study(H, args):
     """The learned absorbing layer at 384^3 (6-cell layer, shipped weights):
     reflection ratios against a 640^3 padded reference run for a source 40 cells
@@ -558,7 +623,14 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 --pmc runs that measure HBM bytes "
                          "per stencil launch (traffic is then null)")
+    ap.add_argument("--batch-sweep", action="store_true",
+                    help="only time every fixed batch tiling on the survey "
+                         "(FP32 and the axis form) and print them as JSON")
     ap.add_argument("--child-pmc", action="store_true", help=argparse.SUPPRESS)
+    # Counter child only: storage form, z chunk, prefetch, workgroups per CU
+    # of a batched survey run.
+    ap.add_argument("--child-batch", type=int, nargs=5, default=None,
+                    help=argparse.SUPPRESS)
     # Counter child only: storage form (field, coefficient) of the stepped
     # rank, for counter runs of the compact kernels.
     ap.add_argument("--child-form", type=int, nargs=2, default=None,
@@ -572,6 +644,15 @@ def main():
         sys.exit(2)
     if args.child_pmc:
         pmc_child(args)
+        return
+    if args.child_batch:
+        batch_child(args)
+        return
+    if args.batch_sweep:
+        pkg = importlib.import_module("distributed-cube-average_amd")
+        VM = importlib.import_module("distributed-cube-average_amd.velocity_model")
+        print(json.dumps({"batch_sweep": batch_sweep(pkg.dc_hip, VM, args)}),
+              flush=True)
         return
 
     traffic = None
