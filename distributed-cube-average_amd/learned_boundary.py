@@ -348,9 +348,11 @@ def main(argv=None):
     def evaluate(mlp, tag):
         scores = {}
         for s in args.eval_seeds:
-            r = reflection(mlp, make_case(s, **ekw), args.eval_steps)
-            log(f"[learned] {tag} eval seed {s}: " +
+            c = make_case(s, **ekw)
+            r = reflection(mlp, c, args.eval_steps)
+            log(f"[learned] {tag} eval seed {s} (source {c.fcut / 3.0:.1f} Hz): " +
                 ", ".join(f"{k} {v:.3e}" for k, v in r.items()))
+            del c
             for k, v in r.items():
                 scores.setdefault(k, []).append(v)
         return scores
