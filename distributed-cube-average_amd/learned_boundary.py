@@ -289,6 +289,74 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
     return out
 
 
+_K8 = np.array([-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0])
+
+
+def damping_layer_1d(freq, c, width, sigma_max, power, d=20.0):
+    """What a damping layer alone can absorb, in one dimension at normal
+    incidence: u_tt = c^2 u_xx - sigma(x) u_t with the 8th-order second
+    difference, leapfrog with the damping centred in time, dt 0.4 of the
+    1-D limit 2 d / (c sqrt(6.5)).  A Ricker of peak `freq` starts at cell
+    100; the layer, sigma = sigma_max ((k+1)/width)^power at its k-th cell,
+    starts at cell 404 with zeros beyond it; the receiver is at cell 200.
+    Returns the energy of (trace - reference trace) over the reference's,
+    the reference having its wall 4000 cells away, over the time the pulse
+    takes to travel 700 cells: the layer's first echo (508 cells) arrives
+    and nothing has bounced twice (about 1300 cells).  With no layer the
+    ratio is one full echo over two pulses of the same energy (the direct
+    one and the echo off the fixed left end): 0.5 where the grid resolves
+    the pulse.  At 3 cells per wavelength (25 Hz at 1500 m/s) dispersion
+    spreads the pulse past the window and the no-layer ratio is 0.36."""
+    dt = 0.4 * 2.0 * d / (c * np.sqrt(6.5))
+    steps = int(700.0 / (c * dt / d))
+    t = np.arange(steps) * dt - 1.2 / freq
+    a = (np.pi * freq * t) ** 2
+    src = (1.0 - 2.0 * a) * np.exp(-a) * dt * dt
+
+    def trace(n_int, w):
+        n = n_int + w + 8
+        sig = np.zeros(n)
+        k = np.arange(w)
+        sig[n_int + 4 + k] = sigma_max * ((k + 1.0) / w) ** power if w else 0.0
+        s = sig[4:-4] * dt / 2.0
+        up, uc = np.zeros(n), np.zeros(n)
+        out = np.empty(steps)
+        for it in range(steps):
+            lap = _K8[0] * uc[4:-4]
+            for j in range(1, 5):
+                lap = lap + _K8[j] * (uc[4 + j:n - 4 + j] + uc[4 - j:n - 4 - j])
+            un = np.zeros(n)
+            un[4:-4] = (2.0 * uc[4:-4] - (1.0 - s) * up[4:-4] +
+                        (c * dt / d) ** 2 * lap) / (1.0 + s)
+            un[100] += src[it]
+            un[n_int + 4 + w:] = 0.0
+            up, uc = uc, un
+            out[it] = uc[200]
+        return out
+
+    ref = trace(4000, 0)
+    got = trace(400, width)
+    return float(np.sum((got - ref) ** 2) / np.sum(ref ** 2))
+
+
+def damping_bound(width, freqs=(10.0, 15.0, 20.0, 25.0),
+                  speeds=(1500.0, 3000.0, 4500.0)):
+    """The best damping_layer_1d over power-law profiles (sigma_max 25-1600
+    /s, powers 1-3) for each frequency and speed: the floor a `width`-cell
+    damping layer cannot go below at normal incidence."""
+    out = []
+    for f in freqs:
+        for c in speeds:
+            best = min((damping_layer_1d(f, c, width, sm, p), sm, p)
+                       for sm in (25, 50, 100, 200, 400, 800, 1600)
+                       for p in (1, 2, 3))
+            out.append({"hz": f, "speed": c, "ratio": best[0],
+                        "sigma_max": best[1], "power": best[2]})
+            log(f"[learned] {width}-cell damping, {f:.0f} Hz, {c:.0f} m/s: "
+                f"best {best[0]:.3e} (sigma_max {best[1]} /s, power {best[2]})")
+    return out
+
+
 def cost(n=384, width=6, steps=50, seed=0):
     """Device ms per step of an n^3-compute domain (random smooth TTI model)
     without and with the learned layer: (plain, learned, layer cells)."""
@@ -316,7 +384,7 @@ def cost(n=384, width=6, steps=50, seed=0):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("cmd", choices=["train", "eval", "cost"])
+    ap.add_argument("cmd", choices=["train", "eval", "cost", "bound"])
     ap.add_argument("--models", type=int, default=64)
     ap.add_argument("--inner", type=int, default=64)
     ap.add_argument("--margin", type=int, default=48)
@@ -357,6 +425,9 @@ def main(argv=None):
                 scores.setdefault(k, []).append(v)
         return scores
 
+    if args.cmd == "bound":
+        damping_bound(args.width)
+        return
     if args.cmd == "cost":
         plain, learned, ncell = cost(args.inner, args.width)
         log(f"[learned] cost {args.inner}^3: plain {plain:.3f} ms/step, with the "
