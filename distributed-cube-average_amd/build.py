@@ -31,7 +31,7 @@ HOST_FLAGS = ["-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c++17",
               "-Wall", "-I" + INC, "-I" + CSRC]
 
 HIP_SOURCES = ["stencil.hip", "halo.hip", "device_data.hip", "program.hip", "batch.hip", "learned.hip",
-               "storage.hip", "noalloc.hip", "model.hip"]
+               "storage.hip", "noalloc.hip", "model.hip", "wall.hip"]
 HOST_SOURCES = ["host_model.cpp"]
 
 

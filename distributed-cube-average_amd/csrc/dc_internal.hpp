@@ -223,6 +223,10 @@ struct State {
   uint32_t *sample_cells;
   // The batch the struct belongs to (dcx_batch_create), if any.
   const void *batch;
+  // Frame condition after each fused step (dcx_set_wall): DCX_WALL_ZERO
+  // leaves the frame as it is, DCX_WALL_ONE_WAY fills it from the outgoing
+  // wave.
+  int wall;
   // dc_propagate calls not yet launched: their boxes and the launch
   // arguments they share (arrays, spacings, dt).  Every other entry point
   // launches them first (state_of), as one launch (flush_pending).
@@ -323,6 +327,9 @@ void fused_step(State *st, float dx, float dy, float dz, float dt,
                 void (*after_stencil)(void *), void *ctx);
 void layer_apply(State *st);
 void layer_free(State *st);
+// The one-way wall's frame fill of the new level (wall.hip; no-op when the
+// struct's wall is DCX_WALL_ZERO).
+void wall_apply(State *st, float dx, float dy, float dz, float dt);
 // Host array (x fastest) <-> pitched device array through a packed device
 // buffer tmp of n floats (NULL: allocated and freed inside).
 void host_to_pitched(const State *st, float *dev, const float *host, float *tmp);

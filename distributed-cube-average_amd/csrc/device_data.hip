@@ -814,6 +814,7 @@ void fused_step(State *st, float dx, float dy, float dz, float dt,
   launch_stencil(st, a);
   if (after_stencil) after_stencil(ctx);
   layer_apply(st);
+  wall_apply(st, dx, dy, dz, dt);
   halo_loopback(st);
   advance(st);
   if (defer) add_deferred_source(st, src_index, src_next);
@@ -920,6 +921,7 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
       launch_stencil(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it + 1], st->stream));
       layer_apply(st);
+      wall_apply(st, dx, dy, dz, dt);
       halo_loopback(st);
       advance(st);
       if (tc) gather(st, *tc, it);
@@ -1065,6 +1067,9 @@ void dcx_set_halo_loopback(dc_device_data *data, int nbox,
   if (st->batch)
     DCX_FAIL("dcx_set_halo_loopback on a batch shot: batched launches carry "
              "no halo exchange (dcx_batch_free first)");
+  if (nbox > 0 && st->wall != DCX_WALL_ZERO)
+    DCX_FAIL("dcx_set_halo_loopback with a one-way wall set: the receive "
+             "boxes lie in the frame the wall fills");
   std::free(st->halo_user);
   st->halo_user = nullptr;
   st->halo_nuser = nbox > 0 ? nbox : 0;

@@ -147,6 +147,7 @@ SIGNATURES = {
     "dcx_mlp_train": (C.c_double, [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                    C.c_float, C.c_longlong, _FP]),
     "dcx_set_learned_layer": (C.c_int, [_DD, C.c_void_p, C.c_int]),
+    "dcx_set_wall": (C.c_int, [_DD, C.c_int]),
     "dcx_learned_sample_step": (C.c_longlong, [
         _DD, _DD, C.c_longlong * 3, C.c_int, C.c_int, C.c_uint, C.c_void_p,
         C.c_float, C.c_float, C.c_float, C.c_float, C.c_longlong, C.c_float,
@@ -678,6 +679,16 @@ def set_learned_layer(d: DeviceData, mlp, width: int) -> int:
                                      width)
     d._layer = mlp if (mlp is not None and width) else None
     return n
+
+
+WALL_ZERO, WALL_ONE_WAY = 0, 1
+
+
+def set_wall(d: DeviceData, mode: int) -> None:
+    """dcx_set_wall: WALL_ZERO keeps the frame at rest (the reference's
+    rigid outer frame), WALL_ONE_WAY fills it each step from the outgoing
+    wave."""
+    load().dcx_set_wall(d.ptr, mode)
 
 
 def learned_sample_step(padded: DeviceData, small: DeviceData, origin, width,

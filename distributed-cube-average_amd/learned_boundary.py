@@ -162,11 +162,15 @@ def make_case(seed, inner=64, width=6, margin=48, d=20.0, cfl_frac=0.4,
 
 def collect(samples: H.Samples, seeds, steps=700, first=60, every=4,
             cells_per_step=1024, mlp=None, rollout=False, taper=False,
-            **case_kw):
+            wall=False, **case_kw):
     """Training samples from the padded runs of the given model seeds.
     rollout: the small domain runs on its own state beside the padded one
     (with `mlp` attached when given) and the samples are its states with the
-    corrections that pull its steps back to the padded run's."""
+    corrections that pull its steps back to the padded run's.  wall: the
+    small domain runs inside the one-way wall (rollout only: a
+    teacher-forced step leaves its frame at rest)."""
+    if wall and not rollout:
+        raise ValueError("the one-way wall is collected in rollout only")
     t0 = time.time()
     for k, seed in enumerate(seeds):
         c = make_case(seed, **case_kw)
@@ -176,6 +180,8 @@ def collect(samples: H.Samples, seeds, steps=700, first=60, every=4,
         small.release_host()
         if rollout and mlp is not None:
             H.set_learned_layer(small, mlp, c.width)
+        if wall:
+            H.set_wall(small, H.WALL_ONE_WAY)
         amp = H.ricker_table(c.dt, steps + 1, c.fcut)
         si = c.pad_index(*c.src)
         ss = c.small_index(*c.src)
@@ -236,11 +242,14 @@ def _receivers(c: Case, gap=4, stride=4):
     return np.array(sorted(pts), np.int64) + np.array(o, np.int64)
 
 
-def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
+def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True,
+               wall=False):
     """Energy ratio of (run - padded) to padded at the interior receivers over
     `steps` steps, for the learned layer, no layer and random-velocity
     shells of the given widths (shells wider than the layer grow the domain
-    outward so that the interior, the receivers and the source stay put)."""
+    outward so that the interior, the receivers and the source stay put).
+    With wall, also the thin domain with the one-way wall alone ("wall") and
+    with the learned layer inside the one-way wall ("learned_wall")."""
     rec = _receivers(c)
     amp = H.ricker_table(c.dt, steps, c.fcut)
 
@@ -264,10 +273,11 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
     ej = float(np.sum(padded.astype(np.float64) ** 2))
     lo_small = tuple(o - STENCIL for o in c.origin)
 
-    def score(coef, lo, net):
+    def score(coef, lo, net, wall_mode=H.WALL_ZERO):
         dd = H.DeviceData(coef, None, c.d, c.d, c.d, c.dt)
         if net is not None:
             H.set_learned_layer(dd, net, c.width)
+        H.set_wall(dd, wall_mode)
         _, idx2, si2 = run(coef, lo)
         tr, ms = dd.run_traces(steps, si2, amp, idx2, graph=graph)
         n = coef[0].shape
@@ -281,6 +291,10 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True):
     small = c.small_coef()
     out["learned"] = score(small, lo_small, mlp)
     out["none"] = score(small, lo_small, None)
+    if wall:
+        out["wall"] = score(small, lo_small, None, H.WALL_ONE_WAY)
+        if mlp is not None:
+            out["learned_wall"] = score(small, lo_small, mlp, H.WALL_ONE_WAY)
     for w in sponge_widths:
         extra = max(0, w - c.width)
         coef = c.sponge_coef(w, extra)
@@ -406,6 +420,17 @@ def main(argv=None):
     ap.add_argument("--taper", type=int, default=1)
     ap.add_argument("--src-depth", type=int, default=None,
                     help="evaluation source this many cells below the low-z layer")
+    ap.add_argument("--wall", action="store_true",
+                    help="also score the one-way wall, alone and around the "
+                         "learned layer")
+    ap.add_argument("--quick", action="store_true",
+                    help="evaluate without the random-velocity shells")
+    ap.add_argument("--out-bias", type=float, default=-3.0,
+                    help="train --wall: initial output bias (damping near "
+                         "zero at the start)")
+    ap.add_argument("--eval-sigma", type=float, default=None,
+                    help="eval: the damping cap to run the shipped network "
+                         "with (default: the one it was trained with)")
     args = ap.parse_args(argv)
     kw = dict(inner=args.inner, width=args.width, margin=args.margin)
     ekw = dict(inner=args.inner, width=args.width, margin=args.eval_margin,
@@ -417,7 +442,8 @@ def main(argv=None):
         scores = {}
         for s in args.eval_seeds:
             c = make_case(s, **ekw)
-            r = reflection(mlp, c, args.eval_steps)
+            r = reflection(mlp, c, args.eval_steps, wall=args.wall,
+                           sponge_widths=(6, 32) if not args.quick else ())
             log(f"[learned] {tag} eval seed {s} (source {c.fcut / 3.0:.1f} Hz): " +
                 ", ".join(f"{k} {v:.3e}" for k, v in r.items()))
             del c
@@ -433,6 +459,39 @@ def main(argv=None):
         log(f"[learned] cost {args.inner}^3: plain {plain:.3f} ms/step, with the "
             f"{args.width}-cell layer ({ncell} cells) {learned:.3f} ms/step; layer "
             f"{learned - plain:.3f} ms = {ncell / (learned - plain) / 1e6:.2f} Gcell/s")
+        return
+    if args.cmd == "train" and args.wall:
+        # Inside the one-way wall a teacher-forced step has nothing to learn
+        # (its frame is at rest, not the wall's), so every pass runs the
+        # small domain on its own state with the current network and fits
+        # the corrections that pull it back to the padded run.
+        per_model = ((args.steps - 60 + 3) // 4) * args.cells
+        samples = H.Samples(per_model * args.models * (1 + args.loop_passes) + 1)
+        params = init_params(0)
+        params[-2:] = args.out_bias
+        mlp = H.Mlp(params, mode=mode, sigma_max=sig)
+        for r in range(1 + args.loop_passes):
+            collect(samples, [2000 * r + k for k in range(args.models)],
+                    steps=args.steps, cells_per_step=args.cells, mlp=mlp,
+                    rollout=True, taper=False, wall=True, **kw)
+            prev = mlp.params()
+            mlp.close()
+            mlp, losses = train(samples, iters=args.iters if r == 0 else
+                                args.loop_iters, batch=args.batch,
+                                lr=args.lr if r == 0 else args.lr * 0.5,
+                                seed=r, params=prev, mode=mode, sigma_max=sig)
+            scores = evaluate(mlp, f"wall pass {r}")
+        save_weights(args.out, mlp.params(), dict(
+            width=args.width, inner=args.inner, margin=args.margin,
+            models=args.models, steps=args.steps, iters=args.iters,
+            loop_passes=args.loop_passes, loop_iters=args.loop_iters,
+            batch=args.batch, lr=args.lr, samples=samples.count(),
+            mode=mode, sigma_max=sig, eval_margin=args.eval_margin, taper=0,
+            wall=1, out_bias=args.out_bias,
+            eval_steps=args.eval_steps, eval_seeds=np.array(args.eval_seeds),
+            final_loss=float(np.mean(losses[-100:])),
+            **{f"eval_{k}": np.array(v) for k, v in scores.items()}))
+        log(f"[learned] wrote {args.out}")
         return
     if args.cmd == "train":
         per_model = ((args.steps - 60 + 3) // 4) * args.cells
@@ -467,8 +526,9 @@ def main(argv=None):
         log(f"[learned] wrote {args.out}")
     else:
         params, meta = load_weights(args.out)
-        mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
-        evaluate(mlp, "shipped")
+        sig = float(meta["sigma_max"]) if args.eval_sigma is None else args.eval_sigma
+        mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=sig)
+        evaluate(mlp, f"shipped (sigma_max {sig})")
 
 
 if __name__ == "__main__":

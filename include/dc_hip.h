@@ -358,6 +358,23 @@ double dcx_mlp_train(dcx_mlp *mlp, const dcx_samples *samples, int n_iter,
  * the number of layer cells. */
 int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *mlp, int width);
 
+/* The frame condition of a whole-domain run.  DCX_WALL_ZERO (the default)
+ * leaves the 4-cell frame beyond the compute region as it is: zero, the
+ * rigid wall of the reference's outer frame (src/boundary.c:7-104).
+ * DCX_WALL_ONE_WAY: after each dcx_step / dcx_run step (after the learned
+ * layer's corrections) the frame cells of the new p and q are filled from
+ * the outgoing wave by the first-order one-way condition, outward one frame
+ * layer at a time:
+ *   u_F^{n+1} = u_I^n + r (u_I^{n+1} - u_F^n),  r = (c dt - h)/(c dt + h),
+ * I being F's neighbour towards the compute region, h the spacing along the
+ * face normal and c^2 = v2px (1 - c_nn) + v2pz c_nn at the last compute cell
+ * (c_nn = cxx, cyy or czz for x, y or z faces).  x faces first (compute
+ * range of y and z), then y faces (all x), then z faces (all x and y).  No
+ * reference counterpart.  FP32 storage, no registered halo loopback, not a
+ * batch shot.  Returns 0. */
+enum { DCX_WALL_ZERO = 0, DCX_WALL_ONE_WAY = 1 };
+int dcx_set_wall(dc_device_data *data, int mode);
+
 /* One training-data step.  `small` is a window of `padded`: its compute
  * region starts at padded cell origin[3] (>= 4 per axis).
  * flags: DCX_SAMPLE_ROLLOUT and/or DCX_SAMPLE_TAPER.

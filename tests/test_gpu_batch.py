@@ -150,14 +150,15 @@ def test_full_size_survey_shots(H, VM):
             assert np.array_equal(bits(g), bits(e)), (k, name)
 
 
-@pytest.mark.parametrize("form", [(0, 0), (0, 4)])
+@pytest.mark.parametrize("form", [(0, 0), (0, 4), (1, 1)])
 def test_whole_survey_batched_equals_shots_alone(H, VM, form):
     """The batching configuration at its workload: all 64 shots of the
     benchmark survey (compute extents U[128,320] x U[128,320] x U[128,256],
     589 M cells per step), each with its own Ricker, in one persistent
-    launch per step with the autotuned tiling and a graph, FP32 and the
-    axis coefficient form.  A few steps (the autotuner's trials included),
-    and every shot's four fields bit-identical to the shot stepped alone."""
+    launch per step with the autotuned tiling and a graph, in FP32, the
+    axis coefficient form and FP16 storage (fields and coefficients).  A few
+    steps (the autotuner's trials included), and every shot's four fields
+    bit-identical to the shot stepped alone."""
     S = VM.layered_survey(seed=0, n_shots=64, steps=40)
     steps = 16
     ws, ds = [], []
