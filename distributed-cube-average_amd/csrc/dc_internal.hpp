@@ -224,9 +224,11 @@ struct State {
   // The batch the struct belongs to (dcx_batch_create), if any.
   const void *batch;
   // Frame condition after each fused step (dcx_set_wall): DCX_WALL_ZERO
-  // leaves the frame as it is, DCX_WALL_ONE_WAY fills it from the outgoing
-  // wave.
+  // leaves the frame as it is, DCX_WALL_ONE_WAY / DCX_WALL_ONE_WAY2 fill it
+  // from the outgoing wave; the second-order condition keeps level n-1 of
+  // the last two compute layers of every face in wall_aux.
   int wall;
+  float *wall_aux;
   // dc_propagate calls not yet launched: their boxes and the launch
   // arguments they share (arrays, spacings, dt).  Every other entry point
   // launches them first (state_of), as one launch (flush_pending).
@@ -327,9 +329,13 @@ void fused_step(State *st, float dx, float dy, float dz, float dt,
                 void (*after_stencil)(void *), void *ctx);
 void layer_apply(State *st);
 void layer_free(State *st);
-// The one-way wall's frame fill of the new level (wall.hip; no-op when the
-// struct's wall is DCX_WALL_ZERO).
+// The one-way wall (wall.hip; no-ops when the struct's wall is
+// DCX_WALL_ZERO): wall_save keeps what the second-order condition needs of
+// level n-1 before the stencil overwrites it, wall_apply fills the frame of
+// the new level after the stencil, wall_free releases the wall's buffer.
+void wall_save(State *st);
 void wall_apply(State *st, float dx, float dy, float dz, float dt);
+void wall_free(State *st);
 // Host array (x fastest) <-> pitched device array through a packed device
 // buffer tmp of n floats (NULL: allocated and freed inside).
 void host_to_pitched(const State *st, float *dev, const float *host, float *tmp);

@@ -572,6 +572,7 @@ void dc_device_data_free(dc_device_data *data) {
   if (st->halo_buf) DCX_CHECK(hipFree(st->halo_buf));
   std::free(st->halo_user);
   layer_free(st);
+  wall_free(st);
   storage_free(st);
   DCX_CHECK(hipEventDestroy(st->ev[0]));
   DCX_CHECK(hipEventDestroy(st->ev[1]));
@@ -811,6 +812,7 @@ void fused_step(State *st, float dx, float dy, float dz, float dt,
   const bool defer = add_next && source_in_send_box(st, src_index);
   if (add_next && !defer) set_source(st, a, src_index, src_next);
   layer_infer(st, a);
+  wall_save(st);
   launch_stencil(st, a);
   if (after_stencil) after_stencil(ctx);
   layer_apply(st);
@@ -917,6 +919,7 @@ static double run_impl(State *st, float dx, float dy, float dz, float dt,
       if (src && !defer && it + 1 < n_steps)
         set_source(st, a, src_index, src_amp[it + 1]);
       layer_infer(st, a);
+      wall_save(st);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it], st->stream));
       launch_stencil(st, a);
       if (evs) DCX_CHECK(hipEventRecord((*evs)[2 * it + 1], st->stream));

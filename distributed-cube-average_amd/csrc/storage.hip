@@ -497,6 +497,11 @@ extern "C" int dcx_set_storage(dc_device_data *data, int field_store,
   if (st->batch)
     DCX_FAIL("dcx_set_storage on a batch shot: free the batch "
              "(dcx_batch_free) before converting its shots");
+  // The wall runs on FP32 storage and holds speeds computed from the FP32
+  // coefficients when it was set.
+  if (st->wall != DCX_WALL_ZERO)
+    DCX_FAIL("dcx_set_storage with a one-way wall set: dcx_set_wall(data, "
+             "DCX_WALL_ZERO) first");
   DCX_CHECK(hipStreamSynchronize(st->stream));
 
   // Bring the fields back to FP32 first if they are compact.

@@ -681,13 +681,13 @@ def set_learned_layer(d: DeviceData, mlp, width: int) -> int:
     return n
 
 
-WALL_ZERO, WALL_ONE_WAY = 0, 1
+WALL_ZERO, WALL_ONE_WAY, WALL_ONE_WAY2 = 0, 1, 2
 
 
 def set_wall(d: DeviceData, mode: int) -> None:
     """dcx_set_wall: WALL_ZERO keeps the frame at rest (the reference's
-    rigid outer frame), WALL_ONE_WAY fills it each step from the outgoing
-    wave."""
+    rigid outer frame), WALL_ONE_WAY / WALL_ONE_WAY2 fill it each step from
+    the outgoing wave by the first- / second-order one-way condition."""
     load().dcx_set_wall(d.ptr, mode)
 
 

@@ -248,8 +248,9 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True,
     `steps` steps, for the learned layer, no layer and random-velocity
     shells of the given widths (shells wider than the layer grow the domain
     outward so that the interior, the receivers and the source stay put).
-    With wall, also the thin domain with the one-way wall alone ("wall") and
-    with the learned layer inside the one-way wall ("learned_wall")."""
+    With wall, also the thin domain inside the first- and second-order
+    one-way walls alone ("wall", "wall2") and with the learned layer inside
+    them ("learned_wall", "learned_wall2")."""
     rec = _receivers(c)
     amp = H.ricker_table(c.dt, steps, c.fcut)
 
@@ -289,12 +290,14 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True,
         return float(np.sum((tr.astype(np.float64) - padded) ** 2) / ej)
 
     small = c.small_coef()
-    out["learned"] = score(small, lo_small, mlp)
+    if mlp is not None:
+        out["learned"] = score(small, lo_small, mlp)
     out["none"] = score(small, lo_small, None)
     if wall:
-        out["wall"] = score(small, lo_small, None, H.WALL_ONE_WAY)
-        if mlp is not None:
-            out["learned_wall"] = score(small, lo_small, mlp, H.WALL_ONE_WAY)
+        for tag, mode in (("wall", H.WALL_ONE_WAY), ("wall2", H.WALL_ONE_WAY2)):
+            out[tag] = score(small, lo_small, None, mode)
+            if mlp is not None:
+                out["learned_" + tag] = score(small, lo_small, mlp, mode)
     for w in sponge_widths:
         extra = max(0, w - c.width)
         coef = c.sponge_coef(w, extra)

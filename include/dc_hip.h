@@ -368,11 +368,21 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *mlp, int width);
  *   u_F^{n+1} = u_I^n + r (u_I^{n+1} - u_F^n),  r = (c dt - h)/(c dt + h),
  * I being F's neighbour towards the compute region, h the spacing along the
  * face normal and c^2 = v2px (1 - c_nn) + v2pz c_nn at the last compute cell
- * (c_nn = cxx, cyy or czz for x, y or z faces).  x faces first (compute
- * range of y and z), then y faces (all x), then z faces (all x and y).  No
- * reference counterpart.  FP32 storage, no registered halo loopback, not a
- * batch shot.  Returns 0. */
-enum { DCX_WALL_ZERO = 0, DCX_WALL_ONE_WAY = 1 };
+ * (c_nn = cxx, cyy or czz for x, y or z faces).  DCX_WALL_ONE_WAY2: the
+ * second-order condition, the first-order operator applied twice, as the
+ * first-order condition on the first-order residual; with
+ * pred(a, b, c) = a + r (b - c) and v_G^m = u_G^m - pred(u_{G-1}^{m-1},
+ * u_{G-1}^m, u_G^{m-1}) (G-1 one cell towards the compute region):
+ *   u_F^{n+1} = pred(u_I^n, u_I^{n+1}, u_F^n) + pred(v_I^n, v_I^{n+1}, v_F^n).
+ * It keeps level n-1 of the last two compute layers in a buffer of the
+ * struct.  c is computed at every face cell when the wall is set (the
+ * coefficients are read-only afterwards; dcx_set_storage is refused while a
+ * one-way wall is set).
+ * x faces first (compute range of y and z), then y faces (all x), then z
+ * faces (all x and y).  No reference counterpart.  FP32 storage, no
+ * registered halo loopback, not a batch shot, at least two compute cells
+ * per axis.  Returns 0. */
+enum { DCX_WALL_ZERO = 0, DCX_WALL_ONE_WAY = 1, DCX_WALL_ONE_WAY2 = 2 };
 int dcx_set_wall(dc_device_data *data, int mode);
 
 /* One training-data step.  `small` is a window of `padded`: its compute

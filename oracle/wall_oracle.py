@@ -30,13 +30,23 @@ def _moved(a, axis):
     return np.moveaxis(a, 2 - axis, 0)
 
 
-def wall_fill(pn, qn, pc, qc, coef, dt, dx, dy, dz):
+def wall_fill(pn, qn, pc, qc, coef, dt, dx, dy, dz, order=1, po=None,
+              qo=None):
     """Fill the 4-cell frame of the new level pn, qn (in place) from the
-    outgoing wave: u_F^{n+1} = u_I^n + r (u_I^{n+1} - u_F^n) outward one
-    layer at a time, r = (c dt - h) / (c dt + h), c^2 = v2px (1 - c_nn) +
-    v2pz c_nn at the last compute cell along the normal.  Each frame layer
-    is one float32 array operation over the face (elementwise, so its
-    results are those of a cell-by-cell loop)."""
+    outgoing wave, outward one layer at a time, r = (c dt - h) / (c dt + h),
+    c^2 = v2px (1 - c_nn) + v2pz c_nn at the last compute cell along the
+    normal.  With the first-order prediction
+      pred(a, b, c) = a + r (b - c)
+    of u_F^{n+1} from a = u_I^n, b = u_I^{n+1}, c = u_F^n:
+    order 1: u_F^{n+1} = pred(u_I^n, u_I^{n+1}, u_F^n);
+    order 2 (po, qo: level n-1 as it was before the step), B B u = 0 written
+    as the first-order condition on the first-order residual
+    v(G, m) = u_G^m - pred(u_{G-1}^{m-1}, u_{G-1}^m, u_G^{m-1}):
+      u_F^{n+1} = pred(u_I^n, u_I^{n+1}, u_F^n)
+                  + pred(v(I, n), v(I, n+1), v(F, n)),
+    G-1 meaning one cell towards the compute region.  Each frame layer is
+    one float32 array operation over the face (elementwise, so its results
+    are those of a cell-by-cell loop)."""
     f32 = np.float32
     S = STENCIL
     n = pn.shape[::-1]                    # (nx, ny, nz)
@@ -60,11 +70,22 @@ def wall_fill(pn, qn, pc, qc, coef, dt, dx, dy, dz):
             c2 = (vx * (f32(1.0) - cn)).astype(f32) + (vz * cn).astype(f32)
             cdt = (np.sqrt(c2.astype(f32)) * dt).astype(f32)
             r = ((cdt - h[axis]).astype(f32) / (cdt + h[axis]).astype(f32)).astype(f32)
-            for new, cur in ((pn, pc), (qn, qc)):
+
+            def pred(a_, b_, c_):
+                return (a_ + (r * (b_ - c_).astype(f32)).astype(f32)).astype(f32)
+            for new, cur, old in ((pn, pc, po), (qn, qc, qo)):
                 N, C = _moved(new, axis), _moved(cur, axis)
                 for k in range(1, S + 1):
                     F = last + dirn * k
                     I = F - dirn
-                    d_ = (N[I][face] - C[F][face]).astype(f32)
-                    N[F][face] = (C[I][face] + (r * d_).astype(f32)).astype(f32)
+                    p1 = pred(C[I][face], N[I][face], C[F][face])
+                    if order == 1:
+                        N[F][face] = p1
+                        continue
+                    O = _moved(old, axis)
+                    I2 = I - dirn
+                    v_in1 = (N[I][face] - pred(C[I2][face], N[I2][face], C[I][face])).astype(f32)
+                    v_in = (C[I][face] - pred(O[I2][face], C[I2][face], O[I][face])).astype(f32)
+                    v_fn = (C[F][face] - pred(O[I][face], C[I][face], O[F][face])).astype(f32)
+                    N[F][face] = (p1 + pred(v_in, v_in1, v_fn)).astype(f32)
     return pn, qn

@@ -1,7 +1,8 @@
 """The one-way wall's CPU restatement (oracle/wall_oracle.py) against what
-the mathematics fixes: the first-order condition reproduces an outgoing
-linear wave exactly on every face, leaves a field at rest in space and time
-unchanged, and its fill order reaches every frame cell."""
+the mathematics fixes: both conditions reproduce outgoing linear and
+quadratic waves exactly on every face, leave a field at rest unchanged, and
+reach every frame cell; the second-order fill is the first-order operator
+applied twice (B B u = 0 evaluated by composing B on random data)."""
 import numpy as np
 import pytest
 
@@ -21,9 +22,11 @@ def const_model(shape, cnn=(0.3, 0.5, 0.2), v2pz=4.0e6, rx=1.4):
     return coef
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("quad", [0.0, 2.0e-5])
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("side", [0, 1])
-def test_outgoing_linear_wave_is_exact(axis, side):
+def test_outgoing_wave_is_exact(axis, side, quad, order):
     nx, ny, nz = 19, 17, 15
     h = (20.0, 15.0, 25.0)
     dt = 1.5e-3
@@ -39,7 +42,9 @@ def test_outgoing_linear_wave_is_exact(axis, side):
     A, B, t = 3.0, 1.0e-3, 0.37
 
     def wave(tt):
-        return np.broadcast_to(A + B * (sgn * X - c * tt), (nz, ny, nx))
+        s_ = sgn * X - c * tt
+        return np.broadcast_to(A + B * s_ + quad * (s_ - 150.0) ** 2 * 1e-3,
+                               (nz, ny, nx))
 
     pc = np.array(wave(t), np.float32)
     qc = np.array(-2.0 * wave(t), np.float32)
@@ -50,7 +55,9 @@ def test_outgoing_linear_wave_is_exact(axis, side):
     frame[S:-S, S:-S, S:-S] = False
     pn[frame] = 1e3
     qn[frame] = -1e3
-    WO.wall_fill(pn, qn, pc, qc, coef, dt, *h)
+    po = np.array(wave(t - dt), np.float32)
+    qo = np.array(-2.0 * wave(t - dt), np.float32)
+    WO.wall_fill(pn, qn, pc, qc, coef, dt, *h, order=order, po=po, qo=qo)
     want = wave(t + dt)
     # The filled frame of this axis's face, the other axes in compute range.
     sl = [slice(S, -S)] * 3
@@ -58,14 +65,20 @@ def test_outgoing_linear_wave_is_exact(axis, side):
     sl = tuple(sl)
     scale = B * h[axis]
     err = np.max(np.abs(pn[sl] - want[sl]))
-    assert err < 2e-3 * scale
-    assert np.max(np.abs(qn[sl] + 2.0 * want[sl])) < 4e-3 * scale
+    tol = 2e-3 if order == 1 else 2e-2   # 8 float32 terms of size ~A
+    assert err < tol * scale
+    assert np.max(np.abs(qn[sl] + 2.0 * want[sl])) < 2 * tol * scale
+    if order == 2:
+        # B_r B_r annihilates every linear wave whatever r (B_r of a linear
+        # wave is a constant, and B_r of a constant is zero), so c is pinned
+        # for order 2 by the composition test below, not here.
+        return
     # A mistake in c (another axis's angle term) misses by far more than the
     # rounding of the exact fill.
     others = [cc for k, cc in enumerate((0.3, 0.5, 0.2)) if k != axis]
     wrong = WO.wall_fill(np.array(pn), np.array(qn), pc, qc,
                          const_model((nz, ny, nx), cnn=(others[0],) * 3), dt,
-                         *h)[0]
+                         *h, order=order, po=po, qo=qo)[0]
     assert np.max(np.abs(wrong[sl] - want[sl])) > 20 * max(err, 1e-7)
 
 
@@ -83,3 +96,62 @@ def test_rest_is_kept_and_every_frame_cell_is_filled():
     WO.wall_fill(pn, qn, pc, qc, coef, 1.5e-3, 20.0, 20.0, 20.0)
     assert np.all(pn == np.float32(0.25))
     assert np.all(qn == np.float32(-0.5))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_rest_is_kept_at_either_order(order):
+    nx, ny, nz = 12, 13, 14
+    coef = const_model((nz, ny, nx))
+    pc = np.full((nz, ny, nx), 0.75, np.float32)
+    pn = pc.copy()
+    pn[:S] = np.nan
+    pn[-S:] = np.nan
+    pn[:, :, :S] = np.nan
+    qc, qn = pc.copy(), pn.copy()
+    WO.wall_fill(pn, qn, pc, qc, coef, 1.5e-3, 20.0, 20.0, 20.0, order=order,
+                 po=pc.copy(), qo=qc.copy())
+    assert np.all(pn == np.float32(0.75)) and np.all(qn == np.float32(0.75))
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_second_order_is_the_first_order_operator_twice(side):
+    """On random data (compute cells at n-1, n, n+1, frame at n-1, n), the
+    second-order fill leaves B(B u) = 0 at every filled x-face cell, B the
+    first-order operator evaluated in float64 on the filled fields; the
+    first-order fill does not."""
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 16, 13, 12
+    h = (20.0, 15.0, 25.0)
+    dt = 1.5e-3
+    coef = const_model((nz, ny, nx))
+    c = float(np.sqrt(np.float32(np.float32(4.0e6 * 1.4) * np.float32(0.7)) +
+                      np.float32(np.float32(4.0e6) * np.float32(0.3))))
+    r = (c * dt - h[0]) / (c * dt + h[0])
+    lev = [rng.standard_normal((nz, ny, nx)).astype(np.float32) for _ in range(3)]
+    old, cur, new = lev
+    new[:, :, :S] = 0.0
+    new[:, :, -S:] = 0.0
+    res = {}
+    for order in (1, 2):
+        pn, qn = new.copy(), new.copy()
+        WO.wall_fill(pn, qn, cur, cur.copy(), coef, dt, *h, order=order,
+                     po=old, qo=old.copy())
+        u = [old.astype(np.float64), cur.astype(np.float64), pn.astype(np.float64)]
+        dirn = 1 if side else -1
+        last = nx - S - 1 if side else S
+
+        def B(m, F):
+            """(B u)(F, m): m indexes u (0 = n-1, 1 = n, 2 = n+1)."""
+            I = F - dirn
+            return (u[m][..., F] - u[m - 1][..., I] -
+                    r * (u[m][..., I] - u[m - 1][..., F]))
+
+        worst = 0.0
+        for k in range(1, S + 1):
+            F = last + dirn * k
+            I = F - dirn
+            bb = B(2, F) - B(1, I) - r * (B(2, I) - B(1, F))
+            worst = max(worst, float(np.max(np.abs(bb[S:-S, S:-S]))))
+        res[order] = worst
+    assert res[2] < 1e-4, res
+    assert res[1] > 0.1, res
