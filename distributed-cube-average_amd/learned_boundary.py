@@ -309,7 +309,7 @@ def reflection(mlp, c: Case, steps, sponge_widths=(6, 32), graph=True,
 _K8 = np.array([-205.0 / 72.0, 8.0 / 5.0, -1.0 / 5.0, 8.0 / 315.0, -1.0 / 560.0])
 
 
-def damping_layer_1d(freq, c, width, sigma_max, power, d=20.0):
+def damping_layer_1d(freq, c, width, sigma_max, power, d=20.0, wall=0):
     """What a damping layer alone can absorb, in one dimension at normal
     incidence: u_tt = c^2 u_xx - sigma(x) u_t with the 8th-order second
     difference, leapfrog with the damping centred in time, dt 0.4 of the
@@ -323,7 +323,10 @@ def damping_layer_1d(freq, c, width, sigma_max, power, d=20.0):
     ratio is one full echo over two pulses of the same energy (the direct
     one and the echo off the fixed left end): 0.5 where the grid resolves
     the pulse.  At 3 cells per wavelength (25 Hz at 1500 m/s) dispersion
-    spreads the pulse past the window and the no-layer ratio is 0.36."""
+    spreads the pulse past the window and the no-layer ratio is 0.36.
+    wall = 1 or 2: the cells beyond the layer are filled each step by the
+    first- or second-order one-way condition of dcx_set_wall (in float64)
+    instead of held at zero."""
     dt = 0.4 * 2.0 * d / (c * np.sqrt(6.5))
     steps = int(700.0 / (c * dt / d))
     t = np.arange(steps) * dt - 1.2 / freq
@@ -338,6 +341,12 @@ def damping_layer_1d(freq, c, width, sigma_max, power, d=20.0):
         s = sig[4:-4] * dt / 2.0
         up, uc = np.zeros(n), np.zeros(n)
         out = np.empty(steps)
+        r = (c * dt - d) / (c * dt + d)
+        L = n_int + 4 + w  # first cell beyond the layer
+
+        def pred(a, b, c_):
+            return a + r * (b - c_)
+
         for it in range(steps):
             lap = _K8[0] * uc[4:-4]
             for j in range(1, 5):
@@ -346,7 +355,16 @@ def damping_layer_1d(freq, c, width, sigma_max, power, d=20.0):
             un[4:-4] = (2.0 * uc[4:-4] - (1.0 - s) * up[4:-4] +
                         (c * dt / d) ** 2 * lap) / (1.0 + s)
             un[100] += src[it]
-            un[n_int + 4 + w:] = 0.0
+            un[L:] = 0.0
+            if wall and n_int < 4000:
+                for F in range(L, n):
+                    p1 = pred(uc[F - 1], un[F - 1], uc[F])
+                    if wall == 2:
+                        v_in1 = un[F - 1] - pred(uc[F - 2], un[F - 2], uc[F - 1])
+                        v_in = uc[F - 1] - pred(up[F - 2], uc[F - 2], up[F - 1])
+                        v_fn = uc[F] - pred(up[F - 1], uc[F - 1], up[F])
+                        p1 += pred(v_in, v_in1, v_fn)
+                    un[F] = p1
             up, uc = uc, un
             out[it] = uc[200]
         return out
@@ -374,9 +392,11 @@ def damping_bound(width, freqs=(10.0, 15.0, 20.0, 25.0),
     return out
 
 
-def cost(n=384, width=6, steps=50, seed=0):
+def cost(n=384, width=6, steps=50, seed=0, walls=False):
     """Device ms per step of an n^3-compute domain (random smooth TTI model)
-    without and with the learned layer: (plain, learned, layer cells)."""
+    without and with the learned layer: (plain, learned, layer cells); with
+    walls, also a dict of ms per step inside the first- and second-order
+    one-way walls (no layer)."""
     rng = np.random.default_rng(seed)
     shape = (n + 2 * STENCIL,) * 3
     vpz, vsv, eps, dlt, theta, phi = random_tti(rng, shape)
@@ -389,6 +409,13 @@ def cost(n=384, width=6, steps=50, seed=0):
     amp = H.ricker_table(dt, steps, 45.0)
     d.run(5, src, amp[:5], graph=True)
     plain = d.run(steps, src, amp, graph=True) / steps
+    wall_ms = {}
+    if walls:
+        for tag, mode in (("wall", H.WALL_ONE_WAY), ("wall2", H.WALL_ONE_WAY2)):
+            H.set_wall(d, mode)
+            d.run(5, src, amp[:5], graph=True)
+            wall_ms[tag] = d.run(steps, src, amp, graph=True) / steps
+        H.set_wall(d, H.WALL_ZERO)
     params, meta = load_weights()
     mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
     ncell = H.set_learned_layer(d, mlp, width)
@@ -396,7 +423,7 @@ def cost(n=384, width=6, steps=50, seed=0):
     learned = d.run(steps, src, amp, graph=True) / steps
     d.close()
     mlp.close()
-    return plain, learned, ncell
+    return (plain, learned, ncell, wall_ms) if walls else (plain, learned, ncell)
 
 
 def main(argv=None):
@@ -456,9 +483,19 @@ def main(argv=None):
 
     if args.cmd == "bound":
         damping_bound(args.width)
+        for order in (1, 2):
+            for f in (10.0, 15.0, 20.0, 25.0):
+                for c in (1500.0, 3000.0, 4500.0):
+                    log(f"[learned] {order}-order one-way wall, no layer, "
+                        f"{f:.0f} Hz, {c:.0f} m/s: "
+                        f"{damping_layer_1d(f, c, 0, 0.0, 1, wall=order):.3e}")
         return
     if args.cmd == "cost":
-        plain, learned, ncell = cost(args.inner, args.width)
+        plain, learned, ncell, wall_ms = cost(args.inner, args.width, walls=True)
+        for tag, ms in wall_ms.items():
+            log(f"[learned] cost {args.inner}^3: plain {plain:.3f} ms/step, inside "
+                f"the {'second' if tag == 'wall2' else 'first'}-order one-way wall "
+                f"{ms:.3f} ms/step")
         log(f"[learned] cost {args.inner}^3: plain {plain:.3f} ms/step, with the "
             f"{args.width}-cell layer ({ncell} cells) {learned:.3f} ms/step; layer "
             f"{learned - plain:.3f} ms = {ncell / (learned - plain) / 1e6:.2f} Gcell/s")
