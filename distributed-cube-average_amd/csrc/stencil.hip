@@ -170,15 +170,21 @@ __device__ __forceinline__ float d1_lds(const float *L, int l, int st) {
 
 // Load/store at a wave-uniform base plus a per-lane unsigned 32-bit byte
 // offset, the form gfx950's global_load/store take with an SGPR base address
-// (one VGPR per offset instead of a 64-bit address pair per load).
+// (one VGPR per offset instead of a 64-bit address pair per load).  The base
+// is cast to the global address space: a batched launch reads its array
+// pointers from device memory, where the compiler cannot tell global from
+// LDS and would emit FLAT accesses, which count against the LDS wait
+// counter, so every wait for the LDS plane would also wait for the
+// prefetched loads.
+#define DCX_GLOBAL __attribute__((address_space(1)))
 template <class T>
 __device__ __forceinline__ T ldb(const void *base, uint32_t boff) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
-                                      boff);
+  return *reinterpret_cast<const DCX_GLOBAL T *>(
+      (const DCX_GLOBAL char *)base + boff);
 }
 template <class T>
 __device__ __forceinline__ void stb(void *base, uint32_t boff, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+  *reinterpret_cast<DCX_GLOBAL T *>((DCX_GLOBAL char *)base + boff) = v;
 }
 
 // Field storage policies (DCX_STORE_*): element size, raw load, decode to
@@ -351,7 +357,7 @@ struct Staged {
 // wave keeps in flight (more VGPRs; same occupancy at 512 threads).  PF = 3:
 // the coefficients and old fields two steps ahead.
 // FS/CS select the field and coefficient storage.
-template <int TX, int TY, int PF, int FS, int CS>
+template <int TX, int TY, int PF, int FS, int CS, bool OB = false>
 __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
                                           const float src_amp) {
   using F = FieldIO<FS>;
@@ -378,10 +384,18 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   static_assert(NAPR <= 2 * NT, "apron must fit two cells per thread");
   static_assert(NDXA <= NT, "D1x apron must fit one cell per thread");
 
-  // One buffer each suffices: every LDS read of plane n happens before the
-  // second barrier of step n, every write of plane n+1 after it.
-  __shared__ float sP[PL], sQ[PL];              // staged plane of p and q
-  __shared__ float sDP[TX * PH], sDQ[TX * PH];  // D1x of p and q, rows -4..TY+3
+  // Two barriers per plane (OB false): one buffer each suffices, since
+  // every LDS read of plane n happens before the second barrier of step n
+  // and every write of plane n+1 after it.  One barrier per plane (OB true,
+  // the axis form): two buffers each, plane n in buffer n & 1; plane n+1 is
+  // staged into the other buffer before step n's barrier, and every read of
+  // a buffer ends before the barrier that precedes its next write.  The
+  // axis form is bound by issue more than by memory, and runs 10% faster
+  // with one barrier; FP32 runs 11% slower.
+  static_assert(!OB || PF == 2, "one barrier per plane needs PF 2");
+  constexpr int NB = OB ? 2 : 1;
+  __shared__ float sP_[NB][PL], sQ_[NB][PL];              // staged planes
+  __shared__ float sDP_[NB][TX * PH], sDQ_[NB][TX * PH];  // D1x, rows -4..TY+3
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -539,6 +553,29 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
     lnx = load_step(zb - 3);
   }
 
+  // Write a staged plane's own and apron cells into LDS buffer k; returns
+  // its own cells' decoded p and q.
+  auto stage = [&](const Staged &sc, int k, float &cp_, float &cq_) {
+    cp_ = F::dec(sc.cp);
+    cq_ = F::dec(sc.cq);
+    sP_[k][lo] = cp_;
+    sQ_[k][lo] = cq_;
+    if (la >= 0) {
+      sP_[k][la] = F::dec(sc.ap);
+      sQ_[k][la] = F::dec(sc.aq);
+    }
+    if (kApron2 && la2 >= 0) {
+      sP_[k][la2] = F::dec(sc.ap2);
+      sQ_[k][la2] = F::dec(sc.aq2);
+    }
+  };
+  // One barrier per plane: plane zb-4 is staged ahead of the loop.
+  float cpn = 0.0f, cqn = 0.0f;
+  if constexpr (OB) {
+    stage(cur, 0, cpn, cqn);
+    __syncthreads();
+  }
+
   // Planes zb-4 .. zb+3 (the first block of 8) fill the z rings without
   // output; every later plane n updates output plane j = n - 4.  The output
   // decision is one per block, not one per plane.
@@ -548,20 +585,21 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
     for (int u = 0; u < 8; ++u) {
       const int n = nb + u;
       if (n >= nend) break;
-      // Stage plane n.
-      const Staged sc = cur;
-      const float cp = F::dec(sc.cp), cq = F::dec(sc.cq);
-      sP[lo] = cp;
-      sQ[lo] = cq;
-      if (la >= 0) {
-        sP[la] = F::dec(sc.ap);
-        sQ[la] = F::dec(sc.aq);
+      // This plane's LDS buffer (every block of 8 starts in buffer 0).
+      const int bi = OB ? (u & 1) : 0;
+      float *const sP = sP_[bi];
+      float *const sQ = sQ_[bi];
+      float *const sDP = sDP_[bi];
+      float *const sDQ = sDQ_[bi];
+      float cp, cq;
+      if constexpr (OB) {
+        cp = cpn;
+        cq = cqn;
+      } else {
+        // Stage plane n.
+        stage(cur, 0, cp, cq);
+        __syncthreads();
       }
-      if (kApron2 && la2 >= 0) {
-        sP[la2] = F::dec(sc.ap2);
-        sQ[la2] = F::dec(sc.aq2);
-      }
-      __syncthreads();
 
       StepLoads L;
       Staged nxt;
@@ -599,6 +637,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
         sDP[dr * TX + dc] = d1_lds(sP, l, 1);
         sDQ[dr * TX + dc] = d1_lds(sQ, l, 1);
       }
+      // One barrier per plane: stage plane n+1 into the other buffer now.
+      if constexpr (OB) stage(nxt, (u + 1) & 1, cpn, cqn);
       __syncthreads();
       const float pxy = d1_lds(sDP, ld, TX) * spf(a.dxy);
       const float qxy = d1_lds(sDQ, ld, TX) * spf(a.dxy);
@@ -767,7 +807,7 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   }
 }
 
-template <int TX, int TY, int PF, int FS, int CS>
+template <int TX, int TY, int PF, int FS, int CS, bool OB = false>
 __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
   // Workgroups are dealt to the 8 XCDs round-robin by linear id.  Mode 1
   // gives each XCD (and its L2) a contiguous run of tiles, x fastest, so the
@@ -783,13 +823,13 @@ __global__ void __launch_bounds__(TX *TY) k_fast(const StepArgs a) {
     const int j = b / 8;
     b = j % a.gx + a.gx * (b % 8 + 8 * (j / a.gx));
   }
-  fast_tile<TX, TY, PF, FS, CS>(a, b, a.src_amp);
+  fast_tile<TX, TY, PF, FS, CS, OB>(a, b, a.src_amp);
 }
 
 // Persistent launch over the tiles of many volumes (shots): workgroup g takes
 // work items g, g + G, g + 2G, ... of the concatenated tile lists; every
 // workgroup runs the same number of items' worth of barriers on its own.
-template <int TX, int TY, int PF, int FS, int CS>
+template <int TX, int TY, int PF, int FS, int CS, bool OB = false>
 __global__ void __launch_bounds__(TX *TY)
     k_fast_batch(const StepArgs *__restrict__ shots,
                  const int64_t *__restrict__ prefix, int nshots,
@@ -808,7 +848,7 @@ __global__ void __launch_bounds__(TX *TY)
     const float amp = (a.src_z >= 0 && a.src_tab != nullptr && it + 1 < n_steps)
                           ? a.src_tab[it + 1]
                           : 0.0f;
-    fast_tile<TX, TY, PF, FS, CS>(a, (int)(w - prefix[k]), amp);
+    fast_tile<TX, TY, PF, FS, CS, OB>(a, (int)(w - prefix[k]), amp);
   }
 }
 
@@ -861,9 +901,19 @@ bool fast_tiling_supported(int tx, int ty) {
   return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && (ty == 8 || ty == 16));
 }
 
+// The axis form runs one barrier per plane (see fast_tile).
+template <int FS, int CS>
+constexpr bool kOneBarrier = FS == DCX_STORE_F32 && CS == DCX_COEF_N16;
+
 template <int TX, int TY, int FS, int CS>
 static void launch_fast_t(const StepArgs &b, dim3 grid, dim3 block,
                           hipStream_t s) {
+  if constexpr (kOneBarrier<FS, CS>) {
+    if (b.deep) {
+      hipLaunchKernelGGL((k_fast<TX, TY, 2, FS, CS, true>), grid, block, 0, s, b);
+      return;
+    }
+  }
   // Three-level prefetch is built for 32x16 FP32 only (it needs the
   // registers the other forms spend on decoding); they take two levels.
   if constexpr (TX == 32 && TY == 16 && FS == DCX_STORE_F32 && CS == DCX_COEF_F32) {
@@ -910,6 +960,14 @@ template <int TX, int TY, int FS, int CS>
 static void launch_batch_t(const StepArgs *shots, const int64_t *prefix,
                            int nshots, int64_t total, int deep, dim3 grid,
                            int it, int n_steps, hipStream_t s) {
+  if constexpr (kOneBarrier<FS, CS>) {
+    if (deep) {
+      hipLaunchKernelGGL((k_fast_batch<TX, TY, 2, FS, CS, true>), grid,
+                         dim3(TX * TY), 0, s, shots, prefix, nshots, total, it,
+                         n_steps);
+      return;
+    }
+  }
   if constexpr (FS == DCX_STORE_F32 && CS == DCX_COEF_F32) {
     if (deep >= 2) {
       hipLaunchKernelGGL((k_fast_batch<TX, TY, 3, FS, CS>), grid,
