@@ -390,8 +390,8 @@ __device__ __forceinline__ void fast_tile(const StepArgs &a, const int b,
   // the axis form): two buffers each, plane n in buffer n & 1; plane n+1 is
   // staged into the other buffer before step n's barrier, and every read of
   // a buffer ends before the barrier that precedes its next write.  The
-  // axis form is bound by issue more than by memory, and runs 10% faster
-  // with one barrier; FP32 runs 11% slower.
+  // compact forms are bound by issue more than by memory and run 5-15%
+  // faster with one barrier (the axis form 10%); plain FP32 runs 11% slower.
   static_assert(!OB || PF == 2, "one barrier per plane needs PF 2");
   constexpr int NB = OB ? 2 : 1;
   __shared__ float sP_[NB][PL], sQ_[NB][PL];              // staged planes
@@ -901,9 +901,9 @@ bool fast_tiling_supported(int tx, int ty) {
   return (tx == 32 && (ty == 16 || ty == 32)) || (tx == 64 && (ty == 8 || ty == 16));
 }
 
-// The axis form runs one barrier per plane (see fast_tile).
+// Every form but plain FP32 runs one barrier per plane (see fast_tile).
 template <int FS, int CS>
-constexpr bool kOneBarrier = FS == DCX_STORE_F32 && CS == DCX_COEF_N16;
+constexpr bool kOneBarrier = !(FS == DCX_STORE_F32 && CS == DCX_COEF_F32);
 
 template <int TX, int TY, int FS, int CS>
 static void launch_fast_t(const StepArgs &b, dim3 grid, dim3 block,
