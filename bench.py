@@ -797,6 +797,9 @@ def main():
                 if "fp32" in batch:
                     mixed["batched_speedup_vs_fp32_shots_alone"] = round(
                         batch[best]["batched"] / batch["fp32"]["alone_default"], 3)
+                    # The same survey batched in plain FP32.
+                    mixed["batched_speedup_vs_fp32_batched"] = round(
+                        batch[best]["batched"] / batch["fp32"]["batched"], 3)
             line["mixed_vs_fp32"] = mixed
     print(json.dumps(line), flush=True)
 
