@@ -143,11 +143,13 @@ struct Pairs {
   float s1, s2, s3, s4, d1, d2, d3, d4;
 };
 
+// The nine points are read from the lowest one up, so that every read is
+// one lane address plus a non-negative constant, which ds_read encodes in
+// its offset field (a negative offset costs an address add per read).
 __device__ __forceinline__ Pairs lds_pairs(const float *L, int l, int st) {
-  const float m1 = L[l - st], p1 = L[l + st];
-  const float m2 = L[l - 2 * st], p2 = L[l + 2 * st];
-  const float m3 = L[l - 3 * st], p3 = L[l + 3 * st];
-  const float m4 = L[l - 4 * st], p4 = L[l + 4 * st];
+  const float *B = L + (l - 4 * st);
+  const float m4 = B[0], m3 = B[st], m2 = B[2 * st], m1 = B[3 * st];
+  const float p1 = B[5 * st], p2 = B[6 * st], p3 = B[7 * st], p4 = B[8 * st];
   return Pairs{p1 + m1, p2 + m2, p3 + m3, p4 + m4,
                p1 - m1, p2 - m2, p3 - m3, p4 - m4};
 }
@@ -163,9 +165,9 @@ __device__ __forceinline__ float d1_of(const Pairs &q) {
 }
 
 __device__ __forceinline__ float d1_lds(const float *L, int l, int st) {
-  return W_L1 * (L[l + st] - L[l - st]) + W_L2 * (L[l + 2 * st] - L[l - 2 * st]) +
-         W_L3 * (L[l + 3 * st] - L[l - 3 * st]) +
-         W_L4 * (L[l + 4 * st] - L[l - 4 * st]);
+  const float *B = L + (l - 4 * st);  // as lds_pairs
+  return W_L1 * (B[5 * st] - B[3 * st]) + W_L2 * (B[6 * st] - B[2 * st]) +
+         W_L3 * (B[7 * st] - B[st]) + W_L4 * (B[8 * st] - B[0]);
 }
 
 // Load/store at a wave-uniform base plus a per-lane unsigned 32-bit byte
