@@ -111,18 +111,22 @@ __global__ void __launch_bounds__(256)
     const uint32_t z = div_by(r, b.dy);
     const uint32_t y = r - z * b.ny;
     const int64_t i = b.base + x + (int64_t)y * b.ys + (int64_t)z * b.zs;
+    // The array pointer comes from LDS, where the compiler cannot tell its
+    // address space: cast it to global, or the access is FLAT and its wait
+    // is also a wait for the LDS reads of the next element's search.
     if (b.esize == 2) {
-      uint16_t *arr = reinterpret_cast<uint16_t *>(b.arr);
+      auto arr = (__attribute__((address_space(1))) uint16_t *)b.arr;
       uint16_t *pk = reinterpret_cast<uint16_t *>(buf);
       if (dir == 0)
         pk[e] = arr[i];
       else
         arr[i] = pk[e];
     } else {
+      auto arr = (__attribute__((address_space(1))) float *)b.arr;
       if (dir == 0)
-        buf[e] = b.arr[i];
+        buf[e] = arr[i];
       else
-        b.arr[i] = buf[e];
+        arr[i] = buf[e];
     }
   }
 }
