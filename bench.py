@@ -604,7 +604,9 @@ def main():
                     help="compute cells per axis of the local block")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tiling", type=int, nargs=3, default=(32, 16, 512))
-    ap.add_argument("--xcd-remap", type=int, default=0)
+    ap.add_argument("--xcd-remap", type=int, default=-1,
+                    help="tile order over the XCDs (-1: the library's "
+                         "default per storage form)")
     ap.add_argument("--prefetch", type=int, default=2, choices=(1, 2, 3))
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")

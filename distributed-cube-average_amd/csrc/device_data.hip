@@ -88,7 +88,11 @@ StepArgs make_step_args(const State *st, const Box &box, float dx, float dy,
   a.src_xy = -1;
   a.src_amp = 0.0f;
   a.zchunk = st->zchunk > 0 ? st->zchunk : auto_zchunk(st, box);
-  a.xcd_remap = st->xcd_remap;
+  // The default tile order (-1) depends on the storage: plain FP32 runs
+  // fastest round robin, every compact form with whole tile rows per XCD.
+  a.xcd_remap = st->xcd_remap >= 0 ? st->xcd_remap
+                : st->fstore == DCX_STORE_F32 && st->cstore == DCX_COEF_F32 ? 0
+                                                                             : 2;
   a.deep = st->deep;
   a.fstore = st->fstore;
   a.cstore = st->cstore;
@@ -505,7 +509,7 @@ State *create_state(const dc_process_t *process, float *tmp) {
   st->zchunk = 0;
   DCX_CHECK(hipDeviceGetAttribute(&st->n_cu, hipDeviceAttributeMultiprocessorCount,
                                   st->device));
-  st->xcd_remap = 0;
+  st->xcd_remap = -1;
   st->deep = 1;
   const size_t abytes = (size_t)st->alloc_floats * sizeof(float);
 
@@ -731,7 +735,7 @@ int dcx_set_prefetch_depth(dc_device_data *data, int planes) {
 int dcx_set_xcd_remap(dc_device_data *data, int mode) {
   State *st = state_of(data);
   const int prev = st->xcd_remap;
-  st->xcd_remap = mode == 2 ? 2 : mode != 0;
+  st->xcd_remap = mode < 0 ? -1 : mode == 2 ? 2 : mode != 0;
   return prev;
 }
 

@@ -367,7 +367,8 @@ class DeviceData:
 
     def set_xcd_remap(self, mode) -> int:
         """dcx_set_xcd_remap: 0 off, 1 (or True) contiguous runs, 2 whole
-        tile rows per XCD; returns the previous mode."""
+        tile rows per XCD, -1 (the default) 0 for plain FP32 and 2 for the
+        compact forms; returns the previous mode."""
         return int(load().dcx_set_xcd_remap(self.ptr, int(mode)))
 
     def set_storage(self, field_store: int, coef_store: int,

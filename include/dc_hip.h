@@ -50,14 +50,15 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
  * Returns the previous depth. */
 int dcx_set_prefetch_depth(dc_device_data *data, int planes);
 
-/* Fast kernel: which XCD (and L2) runs which tile.  0 (default): the
- * hardware's round robin over workgroup ids, so x-neighbour tiles run on
- * different XCDs.  1: each XCD takes a contiguous run of tiles.  2: XCD k
- * takes the whole tile rows r = k mod 8 (when the row count is a multiple
- * of 8), so x neighbours share an L2 while the rows in flight stay one
- * band.  On the 512^3 block modes 1 and 2 are 7-13% slower for FP32 fields
- * and coefficients, and 2-6% faster for the A16 and FP16 forms; results are
- * the same bits in every mode.  Returns the previous mode. */
+/* Fast kernel: which XCD (and L2) runs which tile.  0: the hardware's round
+ * robin over workgroup ids, so x-neighbour tiles run on different XCDs.
+ * 1: each XCD takes a contiguous run of tiles.  2: XCD k takes the whole
+ * tile rows r = k mod 8 (when the row count is a multiple of 8), so x
+ * neighbours share an L2 while the rows in flight stay one band.  -1 (the
+ * default): 0 for FP32 fields and coefficients, 2 for every compact form.
+ * On the 512^3 block mode 2 is 7% slower than 0 for plain FP32 and 1-5.5%
+ * faster for the compact forms; mode 1 is slower for FP32 and the axis form.
+ * Results are the same bits in every mode.  Returns the previous mode. */
 int dcx_set_xcd_remap(dc_device_data *data, int mode);
 
 /* Wait for all queued work of this struct. */

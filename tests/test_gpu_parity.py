@@ -243,29 +243,33 @@ def test_tile_shapes_are_bitwise_equal(H, VM):
             assert np.array_equal(bits(a), bits(b)), k
 
 
+@pytest.mark.parametrize("form", [(0, 0), (0, 4)])
 @pytest.mark.parametrize("shape", [(24, 72, 136), (44, 50, 60)])
-def test_xcd_tile_orders_are_bitwise_equal(H, VM, shape):
+def test_xcd_tile_orders_are_bitwise_equal(H, VM, shape, form):
     """The workgroup-to-tile orders of dcx_set_xcd_remap (0 round robin, 1
-    contiguous runs per XCD, 2 whole tile rows per XCD) only change which
-    XCD runs a tile: a fused 20-step loop with a source gives the same bits.
-    The first grid (128 x 64 x 16 compute cells: 4 x 4 tiles of 32 x 16 in
-    two 8-plane z chunks, 32 tiles in 8 tile rows) is remapped by both modes;
-    the second (52 x 42 x 36, ragged tiles, 30 tiles in 15 rows) suits
-    neither, which then keep the default order."""
+    contiguous runs per XCD, 2 whole tile rows per XCD, -1 the default per
+    storage form) only change which XCD runs a tile: a fused 20-step loop
+    with a source gives the same bits, in plain FP32 and in the axis form
+    (whose default order is 2).  The first grid (128 x 64 x 16 compute
+    cells: 4 x 4 tiles of 32 x 16 in two 8-plane z chunks, 32 tiles in 8
+    tile rows) is remapped by both modes; the second (52 x 42 x 36, ragged
+    tiles, 30 tiles in 15 rows) suits neither, which then keep the default
+    order."""
     w = VM.random_small(shape, seed=16, steps=20)
     amp = w.source_table()
     out = []
-    for mode in (0, 1, 2):
-        say("xcd remap", mode, "shape", shape)
+    for mode in (0, 1, 2, -1):
+        say("xcd remap", mode, "shape", shape, "form", form)
         d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
         assert d.set_tiling(32, 16, 8)
-        assert d.set_xcd_remap(mode) == 0
+        assert d.set_storage(*form)
+        assert d.set_xcd_remap(mode) == -1
         assert d.set_xcd_remap(mode) == mode
         d.run(20, w.src_index, amp, graph=True)
         out.append(d.fields())
         d.close()
     assert np.abs(out[0][1]).max() > 0
-    for k in (1, 2):
+    for k in (1, 2, 3):
         for a, b in zip(out[0], out[k]):
             assert np.array_equal(bits(a), bits(b)), k
 
