@@ -607,7 +607,7 @@ def main():
     ap.add_argument("--xcd-remap", type=int, default=-1,
                     help="tile order over the XCDs (-1: the library's "
                          "default per storage form)")
-    ap.add_argument("--prefetch", type=int, default=2, choices=(1, 2, 3))
+    ap.add_argument("--prefetch", type=int, default=3, choices=(1, 2, 3))
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form-steps", type=int, default=100)

@@ -47,7 +47,7 @@ int dcx_set_tiling(dc_device_data *data, int tile_x, int tile_y, int z_chunk);
  * coefficients at the top of their step; 2 = the staged plane two planes
  * ahead and the coefficients one step ahead; 3 = as 2 with the coefficients
  * and old fields two steps ahead (32x16 FP32 only; other forms use 2).
- * Returns the previous depth. */
+ * The default is 3.  Returns the previous depth. */
 int dcx_set_prefetch_depth(dc_device_data *data, int planes);
 
 /* Fast kernel: which XCD (and L2) runs which tile.  0: the hardware's round

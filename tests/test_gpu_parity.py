@@ -211,7 +211,7 @@ def test_prefetch_depths_are_bitwise_equal(H, VM):
     for depth in (1, 2, 3):
         d = H.DeviceData(w.coef, None, w.dx, w.dy, w.dz, w.dt)
         d.set_tiling(32, 16, 16)
-        assert d.set_prefetch_depth(depth) == 2   # the default
+        assert d.set_prefetch_depth(depth) == 3   # the default
         d.run(30, w.src_index, amp, graph=True)
         out.append(d.fields())
         d.close()
