@@ -361,7 +361,9 @@ int dcx_set_learned_layer(dc_device_data *data, dcx_mlp *mlp, int width);
 
 /* The frame condition of a whole-domain run.  DCX_WALL_ZERO (the default)
  * leaves the 4-cell frame beyond the compute region as it is: zero, the
- * rigid wall of the reference's outer frame (src/boundary.c:7-104).
+ * rigid wall of the reference's outer frame (src/boundary.c:7-104), unless
+ * a wall has filled it (switching back keeps the filled values;
+ * dcx_zero_fields restarts from rest).
  * DCX_WALL_ONE_WAY: after each dcx_step / dcx_run step (after the learned
  * layer's corrections) the frame cells of the new p and q are filled from
  * the outgoing wave by the first-order one-way condition, outward one frame
