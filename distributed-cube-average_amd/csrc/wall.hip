@@ -143,6 +143,25 @@ __global__ void __launch_bounds__(256) k_wall_save(const WallArgs w, int axis) {
   }
 }
 
+// k_wall_save for the x faces, four lanes per row (the two layers of p and
+// of q), for the reason given at k_wall_x.
+__global__ void __launch_bounds__(256) k_wall_save_x(const WallArgs w) {
+  const int q = threadIdx.x & 3;
+  const int layer = q & 1, fl = q >> 1;
+  const int64_t nu = w.n[1] - 2 * STENCIL, nv = w.n[2] - 2 * STENCIL;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 4) + (threadIdx.x >> 2);
+  if (row >= 2 * nu * nv) return;
+  const int side = row >= nu * nv;
+  const int64_t rr = row - (side ? nu * nv : 0);
+  const int64_t iv = rr / nu, iu = rr - iv * nu;
+  const int64_t last = side ? w.n[0] - STENCIL - 1 : STENCIL;
+  const int64_t dir = side ? 1 : -1;
+  const int64_t i = (STENCIL + iu) * w.ys + (STENCIL + iv) * w.zs + last -
+                    dir * (1 - layer);
+  w.aux[w.aoff[0] + (((int64_t)(side * 2 + layer) * 2 + fl) * nv + iv) * nu + iu] =
+      (fl ? w.qn : w.pn)[i];
+}
+
 template <int ORDER>
 __global__ void __launch_bounds__(256) k_wall(const WallArgs w, int axis) {
 #pragma clang fp contract(off)
@@ -210,6 +229,67 @@ __global__ void __launch_bounds__(256) k_wall(const WallArgs w, int axis) {
   }
 }
 
+// The x faces with eight lanes per row along x: lane q holds position j = q
+// of the fill (two idle lanes), so a wavefront's accesses cover 8 rows, each
+// row's six cells in one or two lines, where one lane per row (k_wall, whose
+// lanes run along y) made every lane touch lines of its own.  The outward
+// recurrence passes each new value to the next lane by a shuffle.  Same
+// arithmetic as k_wall, value for value.
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_wall_x(const WallArgs w) {
+#pragma clang fp contract(off)
+  const int q = threadIdx.x & 7;
+  const int64_t nu = w.n[1] - 2 * STENCIL, nv = w.n[2] - 2 * STENCIL;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 8) + (threadIdx.x >> 3);
+  const bool live = row < 2 * nu * nv;
+  const int64_t r0 = live ? row : 0;  // rows past the face load row 0, store nothing
+  const int side = r0 >= nu * nv;
+  const int64_t rr = r0 - (side ? nu * nv : 0);
+  const int64_t iv = rr / nu, iu = rr - iv * nu;
+  const int64_t last = side ? w.n[0] - STENCIL - 1 : STENCIL;
+  const int64_t dir = side ? 1 : -1;
+  const int j = q < 6 ? q : 5;  // idle lanes load what lane 5 loads
+  const int64_t at = (STENCIL + iu) * w.ys + (STENCIL + iv) * w.zs + last +
+                     dir * (j - 1);
+  const float cdt = w.speed[w.soff[0] + (side * nv + iv) * nu + iu] * w.dt;
+  const float r = (cdt - w.h[0]) / (cdt + w.h[0]);
+  auto pred = [r](float a, float b, float c) {
+#pragma clang fp contract(off)
+    return a + r * (b - c);
+  };
+  const int q1 = q > 0 ? q - 1 : 0, q2 = q > 1 ? q - 2 : 0;
+  float *const nw[2] = {w.pn, w.qn};
+  const float *const cu[2] = {w.pc, w.qc};
+#pragma unroll
+  for (int fl = 0; fl < 2; ++fl) {
+    const float c = cu[fl][at];
+    float n = j < 2 ? nw[fl][at] : 0.0f;
+    float o = 0.0f;
+    if constexpr (ORDER == 2) {
+      // Level n-1: saved for the compute layers, still in the frame.
+      o = j < 2 ? w.aux[w.aoff[0] + (((int64_t)(side * 2 + j) * 2 + fl) * nv + iv) * nu + iu]
+                : nw[fl][at];
+    }
+    const float c1 = __shfl(c, q1, 8), c2 = __shfl(c, q2, 8);
+    const float o1 = __shfl(o, q1, 8), o2 = __shfl(o, q2, 8);
+#pragma unroll
+    for (int k = 2; k < 6; ++k) {
+      const float n1 = __shfl(n, k - 1, 8), n2 = __shfl(n, k - 2, 8);
+      if (q == k) {
+        if constexpr (ORDER == 1) {
+          n = pred(c1, n1, c);
+        } else {
+          const float v_in1 = n1 - pred(c2, n2, c1);
+          const float v_in = c1 - pred(o2, c2, o1);
+          const float v_fn = c - pred(o1, c1, o);
+          n = pred(c1, n1, c) + pred(v_in, v_in1, v_fn);
+        }
+      }
+    }
+    if (live && q >= 2 && q < 6) nw[fl][at] = n;
+  }
+}
+
 static WallArgs wall_args(const State *st, float dx, float dy, float dz,
                           float dt) {
   WallArgs w;
@@ -262,7 +342,11 @@ void wall_save(State *st) {
   if (st->wall != DCX_WALL_ONE_WAY2) return;
   check_wall_storage(st);
   const WallArgs w = wall_args(st, 1.f, 1.f, 1.f, 1.f);
-  for (int axis = 0; axis < 3; ++axis) {
+  const int64_t xrows = 2 * (w.n[1] - 2 * STENCIL) * (w.n[2] - 2 * STENCIL);
+  hipLaunchKernelGGL(k_wall_save_x, dim3((unsigned)((xrows + 63) / 64)),
+                     dim3(256), 0, st->stream, w);
+  DCX_CHECK(hipGetLastError());
+  for (int axis = 1; axis < 3; ++axis) {
     hipLaunchKernelGGL(k_wall_save, pass_grid(w.n, axis), dim3(256), 0,
                        st->stream, w, axis);
     DCX_CHECK(hipGetLastError());
@@ -273,7 +357,14 @@ void wall_apply(State *st, float dx, float dy, float dz, float dt) {
   if (st->wall == DCX_WALL_ZERO) return;
   check_wall_storage(st);
   const WallArgs w = wall_args(st, dx, dy, dz, dt);
-  for (int axis = 0; axis < 3; ++axis) {
+  const int64_t xrows = 2 * (w.n[1] - 2 * STENCIL) * (w.n[2] - 2 * STENCIL);
+  const dim3 xgrid((unsigned)((xrows + 31) / 32));
+  if (st->wall == DCX_WALL_ONE_WAY)
+    hipLaunchKernelGGL(k_wall_x<1>, xgrid, dim3(256), 0, st->stream, w);
+  else
+    hipLaunchKernelGGL(k_wall_x<2>, xgrid, dim3(256), 0, st->stream, w);
+  DCX_CHECK(hipGetLastError());
+  for (int axis = 1; axis < 3; ++axis) {
     if (st->wall == DCX_WALL_ONE_WAY)
       hipLaunchKernelGGL(k_wall<1>, pass_grid(w.n, axis), dim3(256), 0,
                          st->stream, w, axis);
