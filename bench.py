@@ -470,20 +470,23 @@ def batch_child(args):
 def learned_study(H, args):
     """The learned absorbing layer at 384^3 (6-cell layer, shipped weights):
     reflection ratios against a 640^3 padded reference run for a source 40 cells
-    below one face over 1000 steps, and the device time per step of the thin
-    domain with and without the layer."""
+    below one face over 1000 steps, beside the zero frame, the reference's
+    random-velocity shells and the one-way walls (alone and around the
+    layer), and the device time per step of the thin domain with and without
+    the layer and inside each wall."""
     LB = importlib.import_module("distributed-cube-average_amd.learned_boundary")
     params, meta = LB.load_weights()
     mlp = H.Mlp(params, mode=int(meta["mode"]), sigma_max=float(meta["sigma_max"]))
     case = LB.make_case(7001, inner=384, width=int(meta["width"]), margin=128,
                         src_depth=40)
-    r = LB.reflection(mlp, case, 1000)
+    r = LB.reflection(mlp, case, 1000, wall=True)
     mlp.close()
     del case
-    plain, learned, ncell = LB.cost(384, int(meta["width"]))
+    plain, learned, ncell, wall_ms = LB.cost(384, int(meta["width"]), walls=True)
     out = {"reflection": {k: float(f"{v:.4g}") for k, v in r.items()},
            "ms_per_step_plain": round(plain, 4),
            "ms_per_step_learned": round(learned, 4), "layer_cells": ncell,
+           "ms_per_step_wall": {k: round(v, 4) for k, v in wall_ms.items()},
            "width": int(meta["width"]), "reference_run": "640^3 padded domain"}
     log(f"[bench] learned layer: {out}")
     return out
